@@ -1,0 +1,66 @@
+// Internal declarations shared by the HIP kernels and the C-ABI host layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tmvp {
+
+constexpr int kN = 256;  // polynomial degree bound
+
+// Which product a launch computes.
+enum Mode : int {
+    kPoly = 0,   // c = a * s                        (one product)
+    kMatVec = 1, // out[i] = sum_j A[i][j] s[j]      (L rows x L terms)
+    kMatVecT = 2,// out[i] = sum_j A[j][i] s[j]      (L rows x L terms)
+    kInner = 3   // v = sum_j b[j] s[j]              (1 row x L terms)
+};
+
+// Fused epilogues applied to the exact sum x.
+enum Epilogue : int {
+    kEpiMod = 0,  // (x [+ old]) mod 2^w
+    kEpiQ2P = 1,  // ((x + h) mod 2^eq) >> (eq - ep)
+    kEpiEnc = 2,  // ((x + h - ((aux & 1) << (ep-1))) mod 2^ep) >> (ep - et)
+    kEpiDec = 3   // ((x + h - ((aux mod 2^et) << (ep-et))) mod 2^ep) >> (ep - 1)
+};
+
+// Everything a product launch needs; passed by value as the kernel argument.
+struct Job {
+    uint16_t *out;
+    const uint16_t *a;   // public operand (A, b or a)
+    const uint16_t *s;   // secret operand
+    const uint16_t *aux; // message bits / scale polynomial (epilogues 2, 3)
+    int *status;         // set to 1 when an MFMA launch meets an out-of-range secret
+    int64_t batch;
+    int w;               // output modulus bits: results are taken mod 2^w
+    int epi;
+    int accumulate;
+    int h;               // h1 (q2p, enc) or h2 (dec)
+    int eq, ep, et;
+};
+
+__host__ __device__ inline constexpr int rows_of(int mode, int l) { return (mode == kMatVec || mode == kMatVecT) ? l : 1; }
+__host__ __device__ inline constexpr int terms_of(int mode, int l) { return mode == kPoly ? 1 : l; }
+
+// Epilogue on one coefficient.  x is the exact sum reduced mod 2^16 or mod
+// 2^32 (both are >= 2^w, so every branch below only reads its low w bits).
+__device__ __forceinline__ uint32_t apply_epilogue(const Job &J, uint32_t x, uint32_t old, uint32_t aux)
+{
+    switch (J.epi) {
+    case kEpiQ2P:
+        return ((x + (uint32_t)J.h) & ((1u << J.eq) - 1u)) >> (J.eq - J.ep);
+    case kEpiEnc:
+        return ((x + (uint32_t)J.h - ((aux & 1u) << (J.ep - 1))) & ((1u << J.ep) - 1u)) >> (J.ep - J.et);
+    case kEpiDec:
+        return ((x + (uint32_t)J.h - ((aux & ((1u << J.et) - 1u)) << (J.ep - J.et))) & ((1u << J.ep) - 1u)) >>
+               (J.ep - 1);
+    default:
+        return (J.accumulate ? x + old : x) & ((1u << J.w) - 1u);
+    }
+}
+
+// Host launchers (each defined next to its kernels).
+hipError_t launch_valu(const Job &J, int mode, int l, hipStream_t stream);
+hipError_t launch_mfma(const Job &J, int mode, int l, hipStream_t stream);
+
+} // namespace tmvp

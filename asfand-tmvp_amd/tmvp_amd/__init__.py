@@ -1,0 +1,345 @@
+"""Python front end of libtmvp.so, the MI355X Saber/Sable TMVP library.
+
+Everything here is plumbing around the C ABI declared in ``include/tmvp.h``:
+the products themselves run in the HIP kernels of ``libtmvp.so``.  Device
+memory, streams and events come straight from the HIP runtime
+(``libamdhip64``) through ctypes; PyTorch is not needed.  If ``libtmvp.so`` has
+not been built, or no gfx950 device is present, every entry point raises -
+there is no CPU fallback.
+
+Reference correspondence (paths relative to the reference root):
+  matvec()     MatVecMul_gpu_shared   TMVP-Saber/SABER-GPU/src/poly.cu:269-376
+  innerprod()  VecVecMul_gpu_shared   TMVP-Saber/SABER-GPU/src/poly.cu:227-248
+               VecVecMul_Inner_gpu    TMVP_SABLE/src/pack.cu:262-286
+  polymul()    one TMVP (convertnegacyclictest .. matvecout_cuda),
+               TMVP-Saber/SABER-GPU/src/poly.cu:1116-1392
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libtmvp.so")
+HIP_RUNTIME = os.environ.get("TMVP_HIP_RUNTIME", "/opt/rocm/lib/libamdhip64.so.7")
+
+N = 256
+
+# parameter sets (include/tmvp.h)
+LIGHTSABER, SABER, FIRESABER, SABLE1, SABLE2, SABLE3 = range(6)
+PARAM_SET_NAMES = {"lightsaber": 0, "saber": 1, "firesaber": 2, "sable1": 3, "sable2": 4, "sable3": 5}
+# backends: reference mode 0 -> MFMA, 1 -> VALU, 2 -> schoolbook (VALU)
+MFMA, VALU, SCHOOLBOOK = 0, 1, 2
+ACCUMULATE = 0x1
+IP_RAW, IP_ENC, IP_DEC = 0, 1, 2
+
+ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "out of memory",
+          -4: "unsupported by this backend", -5: "secret coefficient outside the int8 MFMA range",
+          -6: "no gfx950 device"}
+
+
+class TmvpError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__(f"{where}: {ERRORS.get(code, 'unknown error')} ({code})")
+
+
+def build(force=True):
+    """Compile libtmvp.so in place with hipcc (gfx950)."""
+    cmd = ["make", "-C", PKG_DIR, "-j4"]
+    if force:
+        cmd.append("-B")
+    subprocess.run(cmd, check=True)
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("l", "eq", "ep", "et", "mu", "sable", "h1", "h2", "pk_bytes", "sk_bytes", "ct_bytes")]
+
+
+_hip = None
+_lib = None
+
+# The exported C-ABI symbols, as declared in include/tmvp.h.
+EXPORTS = ("tmvp_init", "tmvp_destroy", "tmvp_ctx_status", "tmvp_strerror", "tmvp_get_params",
+           "tmvp_version", "tmvp_polymul", "tmvp_matvec", "tmvp_innerprod")
+
+
+def load_library(path=LIB_PATH):
+    """Load libtmvp.so (and the HIP runtime it links).  Raises if missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built; run tmvp_amd.build() (hipcc, gfx950)")
+    lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+    P, i, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    sigs = {
+        "tmvp_init": (i, [ctypes.POINTER(P), i]),
+        "tmvp_destroy": (i, [P]),
+        "tmvp_ctx_status": (i, [P, P]),
+        "tmvp_strerror": (ctypes.c_char_p, [i]),
+        "tmvp_get_params": (i, [i, ctypes.POINTER(_Params)]),
+        "tmvp_version": (i, []),
+        "tmvp_polymul": (i, [P, i, P, P, P, i64, i, i, P]),
+        "tmvp_matvec": (i, [P, i, i, i, P, P, P, i64, i, i, P]),
+        "tmvp_innerprod": (i, [P, i, i, i, P, P, P, P, i64, i, P]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+    _lib = lib
+    return lib
+
+
+def hip():
+    """ctypes handle of the HIP runtime (the same one libtmvp.so links)."""
+    global _hip
+    if _hip is None:
+        load_library()
+        h = ctypes.CDLL(HIP_RUNTIME, mode=ctypes.RTLD_GLOBAL)
+        P, i, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+        sigs = {
+            "hipMalloc": (i, [ctypes.POINTER(P), sz]),
+            "hipFree": (i, [P]),
+            "hipMemcpy": (i, [P, P, sz, i]),
+            "hipMemcpyAsync": (i, [P, P, sz, i, P]),
+            "hipMemset": (i, [P, i, sz]),
+            "hipMemsetAsync": (i, [P, i, sz, P]),
+            "hipDeviceSynchronize": (i, []),
+            "hipStreamCreate": (i, [ctypes.POINTER(P)]),
+            "hipStreamDestroy": (i, [P]),
+            "hipStreamSynchronize": (i, [P]),
+            "hipEventCreate": (i, [ctypes.POINTER(P)]),
+            "hipEventDestroy": (i, [P]),
+            "hipEventRecord": (i, [P, P]),
+            "hipEventSynchronize": (i, [P]),
+            "hipEventElapsedTime": (i, [ctypes.POINTER(ctypes.c_float), P, P]),
+            "hipGetDeviceCount": (i, [ctypes.POINTER(i)]),
+            "hipSetDevice": (i, [i]),
+            "hipGetErrorString": (ctypes.c_char_p, [i]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(h, name)
+            f.restype, f.argtypes = res, args
+        _hip = h
+    return _hip
+
+
+def _hipcheck(rc, what):
+    if rc != 0:
+        msg = hip().hipGetErrorString(rc)
+        raise RuntimeError(f"{what} failed: {msg.decode() if msg else rc}")
+
+
+H2D, D2H, D2D = 1, 2, 3
+
+
+class DeviceBuffer:
+    """A hipMalloc'd buffer owned by Python."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        _hipcheck(hip().hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)), "hipMalloc")
+
+    @classmethod
+    def from_numpy(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes)
+        buf.upload(arr)
+        return buf
+
+    def upload(self, arr, stream=None):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        if stream is None:
+            _hipcheck(hip().hipMemcpy(self.ptr, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes, H2D), "hipMemcpy")
+        else:
+            _hipcheck(hip().hipMemcpyAsync(self.ptr, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes, H2D,
+                                           _stream(stream)), "hipMemcpyAsync")
+        return self
+
+    def download(self, dtype=np.uint8, shape=None):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        _hipcheck(hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes, D2H), "hipMemcpy")
+        return out if shape is None else out.reshape(shape)
+
+    def zero(self, stream=None):
+        if stream is None:
+            _hipcheck(hip().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
+        else:
+            _hipcheck(hip().hipMemsetAsync(self.ptr, 0, self.nbytes, _stream(stream)), "hipMemsetAsync")
+        return self
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            hip().hipFree(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        self.handle = ctypes.c_void_p()
+        _hipcheck(hip().hipStreamCreate(ctypes.byref(self.handle)), "hipStreamCreate")
+
+    def synchronize(self):
+        _hipcheck(hip().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+    def destroy(self):
+        if self.handle is not None:
+            hip().hipStreamDestroy(self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        self.handle = ctypes.c_void_p()
+        _hipcheck(hip().hipEventCreate(ctypes.byref(self.handle)), "hipEventCreate")
+
+    def record(self, stream=None):
+        _hipcheck(hip().hipEventRecord(self.handle, _stream(stream)), "hipEventRecord")
+
+    def synchronize(self):
+        _hipcheck(hip().hipEventSynchronize(self.handle), "hipEventSynchronize")
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        _hipcheck(hip().hipEventElapsedTime(ctypes.byref(ms), self.handle, end.handle), "hipEventElapsedTime")
+        return ms.value
+
+    def destroy(self):
+        if self.handle is not None:
+            hip().hipEventDestroy(self.handle)
+            self.handle = None
+
+
+def device_synchronize():
+    _hipcheck(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def get_params(ps):
+    ps = PARAM_SET_NAMES.get(ps, ps)
+    out = _Params()
+    rc = load_library().tmvp_get_params(int(ps), ctypes.byref(out))
+    if rc:
+        raise TmvpError(rc, "tmvp_get_params")
+    return {n: getattr(out, n) for n, _ in _Params._fields_}
+
+
+def _stream(st):
+    if isinstance(st, Stream):
+        return st.handle
+    return st
+
+
+def _ptr(x):
+    if isinstance(x, DeviceBuffer):
+        return x.ptr
+    if x is None:
+        return None
+    return ctypes.c_void_p(int(x))
+
+
+class Context:
+    """A tmvp_ctx on one device.  Methods mirror the C ABI one to one."""
+
+    def __init__(self, device=0):
+        lib = load_library()
+        hip()
+        self.handle = ctypes.c_void_p()
+        rc = lib.tmvp_init(ctypes.byref(self.handle), int(device))
+        if rc:
+            raise TmvpError(rc, "tmvp_init")
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            load_library().tmvp_destroy(self.handle)
+        self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def status(self, stream=None):
+        return load_library().tmvp_ctx_status(self.handle, _stream(stream))
+
+    def check(self, stream=None):
+        rc = self.status(stream)
+        if rc:
+            raise TmvpError(rc, "tmvp_ctx_status")
+
+    def polymul(self, c, a, s, batch, eq_bits=13, backend=MFMA, flags=0, stream=None):
+        rc = load_library().tmvp_polymul(self.handle, backend, _ptr(c), _ptr(a), _ptr(s), int(batch),
+                                         int(eq_bits), int(flags), _stream(stream))
+        if rc:
+            raise TmvpError(rc, "tmvp_polymul")
+
+    def matvec(self, out, A, s, batch, param_set=SABER, backend=MFMA, transpose=False, round_to_p=False,
+               flags=0, stream=None):
+        ps = PARAM_SET_NAMES.get(param_set, param_set)
+        rc = load_library().tmvp_matvec(self.handle, int(ps), backend, int(bool(transpose)), _ptr(out), _ptr(A),
+                                        _ptr(s), int(batch), int(bool(round_to_p)), int(flags), _stream(stream))
+        if rc:
+            raise TmvpError(rc, "tmvp_matvec")
+
+    def innerprod(self, out, b, s, batch, param_set=SABER, backend=MFMA, epilogue=IP_RAW, aux=None, flags=0,
+                  stream=None):
+        ps = PARAM_SET_NAMES.get(param_set, param_set)
+        rc = load_library().tmvp_innerprod(self.handle, int(ps), backend, int(epilogue), _ptr(out), _ptr(b),
+                                           _ptr(s), _ptr(aux), int(batch), int(flags), _stream(stream))
+        if rc:
+            raise TmvpError(rc, "tmvp_innerprod")
+
+
+# ---- numpy conveniences (allocate, copy, run, copy back, synchronise) ----
+
+def run_polymul(ctx, a, s, eq_bits=13, backend=MFMA):
+    a = np.ascontiguousarray(a, np.uint16).reshape(-1, N)
+    s = np.ascontiguousarray(s, np.uint16).reshape(-1, N)
+    batch = a.shape[0]
+    da, ds_, dc = DeviceBuffer.from_numpy(a), DeviceBuffer.from_numpy(s), DeviceBuffer(a.nbytes)
+    ctx.polymul(dc, da, ds_, batch, eq_bits, backend)
+    device_synchronize()
+    ctx.check()
+    return dc.download(np.uint16, (batch, N))
+
+
+def run_matvec(ctx, A, s, param_set=SABER, backend=MFMA, transpose=False, round_to_p=False, out_init=None):
+    l = get_params(param_set)["l"]
+    A = np.ascontiguousarray(A, np.uint16).reshape(-1, l * l * N)
+    s = np.ascontiguousarray(s, np.uint16).reshape(-1, l * N)
+    batch = A.shape[0]
+    dA, ds_ = DeviceBuffer.from_numpy(A), DeviceBuffer.from_numpy(s)
+    dout = DeviceBuffer(batch * l * N * 2)
+    flags = 0
+    if out_init is not None:
+        dout.upload(np.ascontiguousarray(out_init, np.uint16))
+        flags = ACCUMULATE
+    ctx.matvec(dout, dA, ds_, batch, param_set, backend, transpose, round_to_p, flags)
+    device_synchronize()
+    ctx.check()
+    return dout.download(np.uint16, (batch, l, N))
+
+
+def run_innerprod(ctx, b, s, param_set=SABER, backend=MFMA, epilogue=IP_RAW, aux=None):
+    l = get_params(param_set)["l"]
+    b = np.ascontiguousarray(b, np.uint16).reshape(-1, l * N)
+    s = np.ascontiguousarray(s, np.uint16).reshape(-1, l * N)
+    batch = b.shape[0]
+    db, ds_ = DeviceBuffer.from_numpy(b), DeviceBuffer.from_numpy(s)
+    daux = DeviceBuffer.from_numpy(np.ascontiguousarray(aux, np.uint16)) if aux is not None else None
+    dout = DeviceBuffer(batch * N * 2)
+    ctx.innerprod(dout, db, ds_, batch, param_set, backend, epilogue, daux)
+    device_synchronize()
+    ctx.check()
+    return dout.download(np.uint16, (batch, N))

@@ -1,0 +1,141 @@
+/*
+ * tmvp.h - C ABI of the MI355X (gfx950) Saber/Sable TMVP library (libtmvp.so).
+ *
+ * Drop-in for the polynomial-arithmetic hot path of Muhammad-Asfand/asfand-tmvp:
+ * the negacyclic products in Z_q[x]/(x^256+1) behind MatrixVectorMul (A.s and
+ * A^T.s) and InnerProd, with the rounding/shift steps around them.  Reference
+ * citations are relative to the reference root, with
+ *   S/  = TMVP-Saber/SABER-GPU/      SB/ = TMVP_SABLE/
+ *
+ * Conventions (all entry points):
+ *  - Return 0 on success or a negative TMVP_E* code; nothing aborts or exits.
+ *  - Device pointers are plain HIP device addresses, owned by the caller.
+ *    Work is enqueued on `stream` (NULL = the legacy default stream) and the
+ *    call returns without waiting, as the reference's kernel launches do.
+ *  - Layouts are the reference's: polynomials are 256 consecutive uint16
+ *    coefficients; A is [batch][L][L][256] row-major in (i, j); vectors are
+ *    [batch][L][256]; one result polynomial is [batch][256].
+ *  - Secret polynomials may be stored either as uint16 two's complement
+ *    (Saber, S/src/poly.cu:868-871) or masked mod q (Sable,
+ *    SB/src/fips202.cu:646-649): only their value modulo the output modulus
+ *    is used.  The MFMA backend needs that value, centred, to lie in
+ *    [-128, 127] (every CBD secret does); a violation is reported by
+ *    tmvp_ctx_status().
+ *  - Outputs are exact.  Raw products are returned reduced modulo the output
+ *    modulus 2^W (W = EQ for MatrixVectorMul and polymul, EP for InnerProd):
+ *    the reference's uint16 results agree with them mod 2^W, which is the only
+ *    part any later step reads (rounding masks it, S/src/pack.cu:28-32).
+ *  - Thread-safe per context; a context may be used from one host thread at
+ *    a time.
+ */
+#ifndef TMVP_H
+#define TMVP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t without requiring the HIP headers. */
+typedef struct ihipStream_t *tmvp_stream_t;
+
+/* ---- status codes ---- */
+#define TMVP_OK 0
+#define TMVP_EARG (-1)         /* bad argument (null pointer, size, enum) */
+#define TMVP_EHIP (-2)         /* HIP runtime error */
+#define TMVP_ENOMEM (-3)       /* device or host allocation failed */
+#define TMVP_EUNSUPPORTED (-4) /* combination not supported by this backend */
+#define TMVP_ERANGE (-5)       /* a secret coefficient was outside the MFMA range */
+#define TMVP_ENODEV (-6)       /* no gfx950 device */
+
+/* ---- parameter sets (S/include/params.h:6-19, SB/include/SABLE_params.h:6-24) ---- */
+enum tmvp_param_set {
+    TMVP_LIGHTSABER = 0, /* L=2, q=2^13, p=2^10, T=2^3, mu=10 */
+    TMVP_SABER = 1,      /* L=3, q=2^13, p=2^10, T=2^4, mu=8  */
+    TMVP_FIRESABER = 2,  /* L=4, q=2^13, p=2^10, T=2^6, mu=6  */
+    TMVP_SABLE1 = 3,     /* K=2, q=2^11, p=2^9,  T=2^3, mu=2  */
+    TMVP_SABLE2 = 4,     /* K=3, q=2^11, p=2^9,  T=2^5, mu=2  */
+    TMVP_SABLE3 = 5,     /* K=4, q=2^11, p=2^10, T=2^3, mu=2  */
+    TMVP_NUM_PARAM_SETS = 6
+};
+
+/* ---- kernel forms; the reference's `mode` (S/main.cu:25-33):
+ *      0 "TMVP-Tensor" -> TMVP_BACKEND_MFMA (exact int8 split-limb MFMA; the
+ *        reference's fp16 WMMA form, S/src/poly.cu:1155-1205, is inexact)
+ *      1 "TMVP-CUDA"   -> TMVP_BACKEND_VALU
+ *      2 "Schoolbook"  -> TMVP_BACKEND_VALU (same product, one wave per TMVP) */
+enum tmvp_backend {
+    TMVP_BACKEND_MFMA = 0,
+    TMVP_BACKEND_VALU = 1,
+    TMVP_BACKEND_SCHOOLBOOK = 2
+};
+
+/* ---- flags ---- */
+#define TMVP_ACCUMULATE 0x1 /* out = (out + result) mod 2^W, the reference's `r += sum` */
+
+/* ---- InnerProd epilogues ---- */
+enum tmvp_ip_epilogue {
+    TMVP_IP_RAW = 0, /* v mod p */
+    TMVP_IP_ENC = 1, /* ((v + h1 - (m << (EP-1))) mod p) >> (EP-ET); aux = message bits */
+    TMVP_IP_DEC = 2  /* ((v + h2 - (c << (EP-ET))) mod p) >> (EP-1);  aux = ct scale poly */
+};
+
+typedef struct {
+    int l, eq, ep, et, mu, sable, h1, h2;
+    int pk_bytes, sk_bytes, ct_bytes;
+} tmvp_params;
+
+typedef struct tmvp_ctx tmvp_ctx;
+
+/* Library and context management (the reference allocates per call and
+ * never frees, S/src/cuda_kernel.cu:43-94; a context owns its scratch). */
+int tmvp_init(tmvp_ctx **ctx, int device);
+int tmvp_destroy(tmvp_ctx *ctx);
+/* Waits for `stream` and returns TMVP_ERANGE if any MFMA launch since the
+ * last call saw an out-of-range secret coefficient (then clears the flag). */
+int tmvp_ctx_status(tmvp_ctx *ctx, tmvp_stream_t stream);
+const char *tmvp_strerror(int code);
+int tmvp_get_params(int param_set, tmvp_params *out);
+int tmvp_version(void);
+
+/* Raw negacyclic products c[b] = a[b] * s[b] mod (x^256+1, 2^eq_bits).
+ * Replaces one TMVP of the reference: the convertnegacyclictest ->
+ * submatrix_cuda -> matvecp_cuda -> matvecout_cuda chain
+ * (S/src/poly.cu:1116-1129, :1295-1392) and the per-(i,j) triangle loops of
+ * MatVecMul_gpu_shared (S/src/poly.cu:281-284).  eq_bits in [1,16] (VALU) or
+ * [1,14] (MFMA). */
+int tmvp_polymul(tmvp_ctx *ctx, int backend, uint16_t *d_c, const uint16_t *d_a,
+                 const uint16_t *d_s, int64_t batch, int eq_bits, int flags,
+                 tmvp_stream_t stream);
+
+/* MatrixVectorMul: out[b][i] = sum_j A[b][i][j] * s[b][j]            (transpose = 0)
+ *                  out[b][i] = sum_j A[b][j][i] * s[b][j]            (transpose = 1)
+ * Replaces MatVecMul_gpu_shared (S/src/poly.cu:269-376; SB/src/pack.cu:54-163),
+ * launched at S/src/cuda_kernel.cu:129, :411 and SB/src/SABLE_indcpa.cu:186,
+ * :506, and the mode-1 chain S/src/cuda_kernel.cu:389-408.  The reference has
+ * no transposed form (key generation); it is the one its test-vector public
+ * keys were made with.  round_to_p != 0 fuses post_process
+ * (S/src/poly.cu:381-389; SB/src/pack.cu:167-178): out = ((x + h1) mod q) >> (EQ-EP).
+ * Otherwise out = x mod q (TMVP_ACCUMULATE adds into out first). */
+int tmvp_matvec(tmvp_ctx *ctx, int param_set, int backend, int transpose, uint16_t *d_out,
+                const uint16_t *d_A, const uint16_t *d_s, int64_t batch, int round_to_p,
+                int flags, tmvp_stream_t stream);
+
+/* InnerProd: v[b] = sum_j b[b][j] * s[b][j] mod p, then the epilogue.
+ * Replaces VecVecMul_gpu_shared (S/src/poly.cu:227-248, launched
+ * S/src/cuda_kernel.cu:162, :366, :438), VecVecMul_Inner_gpu
+ * (SB/src/pack.cu:262-286) and the mode-1 chain S/src/cuda_kernel.cu:150-159;
+ * the epilogues fuse post_process2 (S/src/poly.cu:391-395; SB/src/pack.cu:290-335)
+ * and post_process3 (S/src/poly.cu:398-403; SB/src/pack.cu:436-442).
+ * d_aux is [batch][256] (message bits for TMVP_IP_ENC, scale polynomial for
+ * TMVP_IP_DEC) and may be NULL for TMVP_IP_RAW. */
+int tmvp_innerprod(tmvp_ctx *ctx, int param_set, int backend, int epilogue, uint16_t *d_out,
+                   const uint16_t *d_b, const uint16_t *d_s, const uint16_t *d_aux,
+                   int64_t batch, int flags, tmvp_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TMVP_H */
