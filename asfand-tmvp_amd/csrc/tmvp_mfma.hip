@@ -15,24 +15,27 @@
 // (S/src/poly.cu:1116-1129) and multiplies three padded 128 x 128 fp16 tiles
 // (S/src/poly.cu:1155-1205), which is inexact for q = 2^13.
 //
-// Exactness.  The secret is centred mod 2^w into int8 (CBD values are at most
-// 5 in magnitude).  The public coefficient (mod 2^w, w <= 14) is split into
-// limbs a = lo + 128 hi with lo in [0,127], hi in [0,127]; the negated copies
-// stay inside int8.  Each limb product accumulates in int32
-// (|sum| <= 4 * 256 * 127 * 127 < 2^31); out = lo + 128 hi is exact mod 2^32.
+// Exactness.  The secret is used centred mod 2^w as an int8 (CBD values are
+// at most 5 in magnitude; for w >= 8 that int8 is simply the low byte).  The
+// public coefficient (mod 2^w, w <= 14) is split into limbs a = lo + 128 hi,
+// lo and hi in [0, 127]; the negated copies stay inside int8.  Each limb
+// product accumulates exactly in int32 (|sum| <= 4*256*127*127 < 2^31) and
+// out = lo + 128 hi is exact mod 2^32.
 //
 // Mapping onto v_mfma_i32_16x16x64_i8 (M=16, N=16, K=64):
-//   M  = 16 rows i of one half (h) of a 32-row block row,
-//   K  = 64 = two consecutive D tiles x 32 jj  (four K-chunks p cover D = 0..7),
-//   N  = 16 = 8 block columns R x 2 limbs.
-// The A operand (Toeplitz tiles of s) is built once per secret polynomial
+//   M = 16 rows i of one half (h) of a 32-row block row,
+//   K = 64 = two consecutive D tiles x 32 jj (four K-chunks p cover D = 0..7),
+//   N = 16 = 8 block columns R x 2 limbs.
+// The A operand (Toeplitz tiles of s) is loaded once per secret polynomial
 // into 32 VGPRs and reused by every row and both limbs; the B operand is one
-// aligned 16-byte LDS read per MFMA pair (it feeds both halves h).  A lane of
-// the A operand needs 16 consecutive band bytes at an unaligned offset whose
-// misalignment is fixed per lane, so it is assembled from five dwords with
-// v_alignbyte.  K indices are laid out in reversed jj order inside each
-// 16-byte group (the E blocks are stored reversed) so that this window is
-// ascending in memory.
+// aligned 16-byte LDS read per MFMA pair (it feeds both halves h).
+//
+// An A-operand lane needs 16 consecutive band bytes at an offset whose
+// misalignment (i + 1) mod 4 is fixed per lane.  The band is therefore stored
+// four times, shifted by 0..3 bytes, and every lane reads its own copy with
+// dword-aligned LDS loads.  K indices run in reversed jj order inside each
+// 16-byte group (the E blocks are stored byte-reversed) so that these windows
+// ascend in memory.
 //
 // Launch: 256 threads = 4 waves, one instance per wave (all rows and terms),
 // every wave with its own LDS slice.
@@ -41,114 +44,166 @@
 namespace tmvp {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kEBlock = 32;               // bytes per E block (32 coefficients, one limb)
-constexpr int kEPoly = 16 * kEBlock;      // e = 0..15 (e = 0 unused)
-constexpr int kBand = 320;                // bytes per secret band (288 used)
+// Per-halfword add of two packed uint16 pairs (wraps mod 2^16).
+__device__ __forceinline__ uint32_t add16x2(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) + __builtin_bit_cast(u16x2, b));
+}
 
-__device__ __forceinline__ uint32_t bswap32(uint32_t v) { return __builtin_bswap32(v); }
+constexpr int kEBlock = 32;           // bytes per E block (32 coefficients, one limb)
+constexpr int kEPoly = 16 * kEBlock;  // blocks e = 0..15 (e = 0 unused)
+constexpr int kBand = 288;            // bytes of one band copy: X[x] = ts(x - 32), x < 288
 
 // Bytewise negation of four bytes that are each <= 0x80.
-__device__ __forceinline__ uint32_t neg_bytes_small(uint32_t v) { return (0x80808080u - v) ^ 0x80808080u; }
+__device__ __forceinline__ uint32_t neg_small(uint32_t v) { return (0x80808080u - v) ^ 0x80808080u; }
+// Bytewise two's-complement negation of four arbitrary bytes.
+__device__ __forceinline__ uint32_t neg_bytes(uint32_t v)
+{
+    const uint32_t y = ~v;
+    return ((y & 0x7f7f7f7fu) + 0x01010101u) ^ (y & 0x80808080u);
+}
+__device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) { return __builtin_amdgcn_perm(hi, lo, sel); }
 
-template <int L, int MODE>
+template <int EPI>
+__device__ __forceinline__ uint32_t epi(const Job &J, uint32_t x, uint32_t old, uint32_t aux)
+{
+    if (EPI == kEpiQ2P) return ((x + (uint32_t)J.h) & ((1u << J.eq) - 1u)) >> (J.eq - J.ep);
+    if (EPI == kEpiEnc)
+        return ((x + (uint32_t)J.h - ((aux & 1u) << (J.ep - 1))) & ((1u << J.ep) - 1u)) >> (J.ep - J.et);
+    if (EPI == kEpiDec)
+        return ((x + (uint32_t)J.h - ((aux & ((1u << J.et) - 1u)) << (J.ep - J.et))) & ((1u << J.ep) - 1u)) >>
+               (J.ep - 1);
+    return (x + old) & ((1u << J.w) - 1u);
+}
+
+template <int L, int MODE, int EPI>
 __global__ __launch_bounds__(256) void mfma_kernel(Job J)
 {
     constexpr int NR = rows_of(MODE, L);
     constexpr int NT = terms_of(MODE, L);
-    constexpr int E_BYTES = NR * NT * 2 * kEPoly;
-    constexpr int WAVE_BYTES = E_BYTES + NT * kBand;
+    constexpr int NPOLY = NR * NT;             // public polynomials per instance
+    constexpr int CHUNKS = NPOLY * 32;         // 8-coefficient chunks per instance
+    constexpr int E_BYTES = NPOLY * 2 * kEPoly;
+    constexpr int WAVE_BYTES = E_BYTES + NT * 4 * kBand;
     __shared__ __attribute__((aligned(16))) uint8_t lds[4 * WAVE_BYTES];
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t inst = (int64_t)blockIdx.x * 4 + wave;
-    const bool valid = inst < J.batch;
+    if (inst >= J.batch) return; // wave-uniform; no barrier follows for this wave's partners
     uint8_t *E = lds + wave * WAVE_BYTES;
     uint8_t *X = E + E_BYTES;
     const uint32_t wmask = (1u << J.w) - 1u;
 
-    if (valid) {
-        // ---- public polynomials -> reversed limb blocks E (and negated copies)
-        const uint16_t *abase = (MODE == kPoly) ? J.a + inst * kN
-                              : (MODE == kInner) ? J.a + inst * L * kN
-                                                 : J.a + inst * L * L * kN;
-        for (int c = lane; c < NR * NT * 32; c += 64) {
-            const int pq = c >> 5;          // polynomial index in memory order
-            const int k0 = (c & 31) * 8;    // first of 8 coefficients
+    // ---- public polynomials -> byte-reversed limb blocks E[e] (+ negated copies)
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(J.a + inst * (NPOLY * kN));
+        const uint32_t lo2 = (wmask < 127u ? wmask : 127u) * 0x00010001u;
+        const uint32_t hi2 = (wmask >> 7) * 0x00010001u;
+        const int blk = (lane & 31) >> 2;              // 32-coefficient block of this chunk
+        const int pos = 24 - 8 * (lane & 3);           // byte position inside the reversed block
+        uint4 v[(CHUNKS + 63) / 64];
+#pragma unroll
+        for (int it = 0; it < (CHUNKS + 63) / 64; it++) {
+            const int c = lane + 64 * it;
+            if (CHUNKS % 64 == 0 || c < CHUNKS) v[it] = src[c];
+        }
+#pragma unroll
+        for (int it = 0; it < (CHUNKS + 63) / 64; it++) {
+            const int c = lane + 64 * it;
+            if (CHUNKS % 64 != 0 && c >= CHUNKS) continue;
+            const int pq = c >> 5;                     // memory order of the polynomial
             int r = 0, j = pq;
             if (MODE == kMatVec) { r = pq / L; j = pq % L; }
             if (MODE == kMatVecT) { r = pq % L; j = pq / L; }
-            const uint4 v = *(const uint4 *)(abase + pq * kN + k0);
-            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
-            uint32_t lo[2] = {0, 0}, hi[2] = {0, 0};
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const uint32_t x = ((w4[u >> 1] >> (16 * (u & 1))) & 0xffffu) & wmask;
-                lo[u >> 2] |= (x & 127u) << (8 * (u & 3));
-                hi[u >> 2] |= (x >> 7) << (8 * (u & 3));
-            }
-            const int blk = k0 >> 5, pos = 24 - (k0 & 31);
-            uint8_t *slot = E + (r * NT + j) * 2 * kEPoly;
-            const uint2 plo = make_uint2(bswap32(lo[1]), bswap32(lo[0]));
-            const uint2 phi = make_uint2(bswap32(hi[1]), bswap32(hi[0]));
-            *(uint2 *)(slot + (blk + 8) * kEBlock + pos) = plo;
-            *(uint2 *)(slot + kEPoly + (blk + 8) * kEBlock + pos) = phi;
+            const uint32_t l0 = v[it].x & lo2, l1 = v[it].y & lo2, l2 = v[it].z & lo2, l3 = v[it].w & lo2;
+            const uint32_t h0 = (v[it].x >> 7) & hi2, h1 = (v[it].y >> 7) & hi2;
+            const uint32_t h2 = (v[it].z >> 7) & hi2, h3 = (v[it].w >> 7) & hi2;
+            // coefficient u of the chunk goes to byte 7 - u
+            const uint2 plo = make_uint2(perm(l3, l2, 0x00020406u), perm(l1, l0, 0x00020406u));
+            const uint2 phi = make_uint2(perm(h3, h2, 0x00020406u), perm(h1, h0, 0x00020406u));
+            uint8_t *slot = E + (r * NT + j) * 2 * kEPoly + pos;
+            *(uint2 *)(slot + (blk + 8) * kEBlock) = plo;
+            *(uint2 *)(slot + kEPoly + (blk + 8) * kEBlock) = phi;
             if (blk > 0) {
-                *(uint2 *)(slot + blk * kEBlock + pos) = make_uint2(neg_bytes_small(plo.x), neg_bytes_small(plo.y));
-                *(uint2 *)(slot + kEPoly + blk * kEBlock + pos) =
-                    make_uint2(neg_bytes_small(phi.x), neg_bytes_small(phi.y));
+                *(uint2 *)(slot + blk * kEBlock) = make_uint2(neg_small(plo.x), neg_small(plo.y));
+                *(uint2 *)(slot + kEPoly + blk * kEBlock) = make_uint2(neg_small(phi.x), neg_small(phi.y));
             }
         }
-        // ---- secret polynomials -> int8 bands X[x] = ts(x - 32), x in [0, 288)
-        const uint16_t *sbase = (MODE == kPoly) ? J.s + inst * kN : J.s + inst * L * kN;
-        const uint32_t half = 1u << (J.w - 1);
-        bool bad = false;
+    }
+    // ---- secret polynomials -> int8 band X[x] = ts(x - 32), stored 4x shifted:
+    //      copy c holds X[x + c] at byte x.
+    {
+        const uint2 *src = reinterpret_cast<const uint2 *>(J.s + inst * (NT * kN));
+        uint32_t bad = 0;
 #pragma unroll
         for (int j = 0; j < NT; j++) {
-            const uint2 v = *(const uint2 *)(sbase + j * kN + 4 * lane);
-            const uint32_t w2[2] = {v.x, v.y};
-            uint32_t pos = 0, neg = 0;
+            const uint2 sv = src[j * 64 + lane];
+            uint32_t pos;
+            if (J.w >= 8) {
+                pos = perm(sv.y, sv.x, 0x06040200u); // low bytes = int8 of the centred value
+            } else {
+                const int sh = 32 - J.w;
+                const uint32_t cs[4] = {sv.x & 0xffffu, sv.x >> 16, sv.y & 0xffffu, sv.y >> 16};
+                pos = 0;
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t x = ((w2[u >> 1] >> (16 * (u & 1))) & 0xffffu) & wmask;
-                const int sv = (x >= half) ? (int)x - (int)(wmask + 1u) : (int)x;
-                bad |= (sv < -127) | (sv > 127);
-                pos |= (uint32_t)(sv & 0xff) << (8 * u);
-                neg |= (uint32_t)((-sv) & 0xff) << (8 * u);
+                for (int u = 0; u < 4; u++) pos |= ((uint32_t)(((int)(cs[u] << sh)) >> sh) & 0xffu) << (8 * u);
             }
-            uint8_t *band = X + j * kBand;
+            // range check: centred value in [-127, 127]  <=>  ((s + 127) mod 2^w) + 1 <= 255
+            const uint32_t wm2 = wmask * 0x00010001u;
+            const uint32_t t0 = add16x2(sv.x, 0x007f007fu) & wm2;
+            const uint32_t t1 = add16x2(sv.y, 0x007f007fu) & wm2;
+            bad |= (add16x2(t0, 0x00010001u) | add16x2(t1, 0x00010001u)) & 0xff00ff00u;
+            const uint32_t neg = neg_bytes(pos);
+            // dword k of X: k = lane + 8 holds pos of this lane; k = lane - 56 (lanes 56..63) holds neg
+            uint32_t nxt_pos = __shfl_down(pos, 1);
+            uint32_t nxt_neg = __shfl_down(neg, 1);
+            const uint32_t first_pos = __shfl(pos, 0);
+            if (lane == 63) {
+                nxt_pos = 0u;        // X[288..291] (never read as data)
+                nxt_neg = first_pos; // dword 8 follows dword 7
+            }
+            uint8_t *band = X + j * 4 * kBand;
             *(uint32_t *)(band + 32 + 4 * lane) = pos;
-            if (lane >= 56) *(uint32_t *)(band + 4 * lane - 224) = neg;
-            if (lane < 8) *(uint32_t *)(band + 288 + 4 * lane) = 0u; // pad read by the last window
+#pragma unroll
+            for (int c = 1; c < 4; c++)
+                *(uint32_t *)(band + c * kBand + 32 + 4 * lane) = __builtin_amdgcn_alignbyte(nxt_pos, pos, c);
+            if (lane >= 56) {
+                *(uint32_t *)(band + 4 * lane - 224) = neg;
+#pragma unroll
+                for (int c = 1; c < 4; c++)
+                    *(uint32_t *)(band + c * kBand + 4 * lane - 224) = __builtin_amdgcn_alignbyte(nxt_neg, neg, c);
+            }
         }
         if (bad) *J.status = 1;
     }
-    __syncthreads();
-    if (!valid) return;
+    // The slice is private to this wave: LDS operations of one wave complete in
+    // order, so a wave-level fence is enough before reading it back.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
     const int il = lane & 15, g = lane >> 4;
-    const int n = lane & 15, R = n & 7, limb = n >> 3;
+    const int R = lane & 7, limb = (lane >> 3) & 1;
     const int sh = (il + 1) & 3;
 
     v4i acc[NR][2];
 #pragma unroll
     for (int r = 0; r < NR; r++) acc[r][0] = acc[r][1] = v4i{0, 0, 0, 0};
 
-#pragma unroll 1
+#pragma unroll
     for (int j = 0; j < NT; j++) {
         // A operand: Toeplitz tiles of s_j for the four K-chunks and both halves.
-        const uint32_t *X32 = (const uint32_t *)(X + j * kBand);
+        const uint8_t *copy = X + j * 4 * kBand + sh * kBand;
         v4i afr[4][2];
 #pragma unroll
         for (int p = 0; p < 4; p++) {
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const int D = 2 * p + (g >> 1);
-                const int x0 = 32 * D + 16 * h - 16 * (g & 1) + il + 17;
-                const int q = x0 >> 2;
-                const uint32_t w0 = X32[q], w1 = X32[q + 1], w2 = X32[q + 2], w3 = X32[q + 3], w4 = X32[q + 4];
-                afr[p][h] = v4i{(int)__builtin_amdgcn_alignbyte(w1, w0, sh), (int)__builtin_amdgcn_alignbyte(w2, w1, sh),
-                                (int)__builtin_amdgcn_alignbyte(w3, w2, sh), (int)__builtin_amdgcn_alignbyte(w4, w3, sh)};
+                const int x0 = 32 * (2 * p + (g >> 1)) + 16 * h - 16 * (g & 1) + il + 17;
+                const uint32_t *w = reinterpret_cast<const uint32_t *>(copy + (x0 - sh));
+                afr[p][h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
             }
         }
 #pragma unroll
@@ -164,11 +219,11 @@ __global__ __launch_bounds__(256) void mfma_kernel(Job J)
         }
     }
 
-    // ---- epilogue: lanes n < 8 own block column R; rows i = 16h + 4g + v.
+    // ---- epilogue: lanes with limb 0 own block column R; rows i = 16h + 4g + v.
+    uint16_t *obase = J.out + inst * (NR * kN);
+    const uint16_t *abase = (EPI == kEpiEnc || EPI == kEpiDec) ? J.aux + inst * kN : nullptr;
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-        uint16_t *op = J.out + (inst * NR + r) * kN;
-        const uint16_t *auxp = J.aux ? J.aux + inst * kN : nullptr;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             uint32_t val[4];
@@ -180,24 +235,35 @@ __global__ __launch_bounds__(256) void mfma_kernel(Job J)
             if (limb == 0) {
                 const int m0 = 32 * R + 16 * h + 4 * g;
                 uint2 oldv = make_uint2(0u, 0u), auxv = make_uint2(0u, 0u);
-                if (J.accumulate && J.epi == kEpiMod) oldv = *(const uint2 *)(op + m0);
-                if (auxp) auxv = *(const uint2 *)(auxp + m0);
-                const uint32_t r0 = apply_epilogue(J, val[0], oldv.x & 0xffff, auxv.x & 0xffff);
-                const uint32_t r1 = apply_epilogue(J, val[1], oldv.x >> 16, auxv.x >> 16);
-                const uint32_t r2 = apply_epilogue(J, val[2], oldv.y & 0xffff, auxv.y & 0xffff);
-                const uint32_t r3 = apply_epilogue(J, val[3], oldv.y >> 16, auxv.y >> 16);
-                *(uint2 *)(op + m0) = make_uint2(r0 | (r1 << 16), r2 | (r3 << 16));
+                if (EPI == kEpiMod && J.accumulate) oldv = *(const uint2 *)(obase + r * kN + m0);
+                if (EPI == kEpiEnc || EPI == kEpiDec) auxv = *(const uint2 *)(abase + m0);
+                const uint32_t o0 = epi<EPI>(J, val[0], oldv.x & 0xffffu, auxv.x & 0xffffu);
+                const uint32_t o1 = epi<EPI>(J, val[1], oldv.x >> 16, auxv.x >> 16);
+                const uint32_t o2 = epi<EPI>(J, val[2], oldv.y & 0xffffu, auxv.y & 0xffffu);
+                const uint32_t o3 = epi<EPI>(J, val[3], oldv.y >> 16, auxv.y >> 16);
+                *(uint2 *)(obase + r * kN + m0) = make_uint2(o0 | (o1 << 16), o2 | (o3 << 16));
             }
         }
     }
 }
 
-template <int L, int MODE>
+template <int L, int MODE, int EPI>
 static hipError_t go(const Job &J, hipStream_t stream)
 {
     const int64_t blocks = (J.batch + 3) / 4;
-    hipLaunchKernelGGL((mfma_kernel<L, MODE>), dim3((unsigned)blocks), dim3(256), 0, stream, J);
+    hipLaunchKernelGGL((mfma_kernel<L, MODE, EPI>), dim3((unsigned)blocks), dim3(256), 0, stream, J);
     return hipGetLastError();
+}
+
+template <int MODE, int EPI>
+static hipError_t by_l(const Job &J, int l, hipStream_t stream)
+{
+    switch (l) {
+    case 2: return go<2, MODE, EPI>(J, stream);
+    case 3: return go<3, MODE, EPI>(J, stream);
+    case 4: return go<4, MODE, EPI>(J, stream);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_mfma(const Job &J, int mode, int l, hipStream_t stream)
@@ -205,27 +271,20 @@ hipError_t launch_mfma(const Job &J, int mode, int l, hipStream_t stream)
     if (J.batch <= 0) return hipSuccess;
     switch (mode) {
     case kPoly:
-        return go<1, kPoly>(J, stream);
+        if (J.epi == kEpiMod) return go<1, kPoly, kEpiMod>(J, stream);
+        break;
     case kMatVec:
-        switch (l) {
-        case 2: return go<2, kMatVec>(J, stream);
-        case 3: return go<3, kMatVec>(J, stream);
-        case 4: return go<4, kMatVec>(J, stream);
-        }
+        if (J.epi == kEpiMod) return by_l<kMatVec, kEpiMod>(J, l, stream);
+        if (J.epi == kEpiQ2P) return by_l<kMatVec, kEpiQ2P>(J, l, stream);
         break;
     case kMatVecT:
-        switch (l) {
-        case 2: return go<2, kMatVecT>(J, stream);
-        case 3: return go<3, kMatVecT>(J, stream);
-        case 4: return go<4, kMatVecT>(J, stream);
-        }
+        if (J.epi == kEpiMod) return by_l<kMatVecT, kEpiMod>(J, l, stream);
+        if (J.epi == kEpiQ2P) return by_l<kMatVecT, kEpiQ2P>(J, l, stream);
         break;
     case kInner:
-        switch (l) {
-        case 2: return go<2, kInner>(J, stream);
-        case 3: return go<3, kInner>(J, stream);
-        case 4: return go<4, kInner>(J, stream);
-        }
+        if (J.epi == kEpiMod) return by_l<kInner, kEpiMod>(J, l, stream);
+        if (J.epi == kEpiEnc) return by_l<kInner, kEpiEnc>(J, l, stream);
+        if (J.epi == kEpiDec) return by_l<kInner, kEpiDec>(J, l, stream);
         break;
     }
     return hipErrorInvalidValue;

@@ -87,6 +87,18 @@ def cpu_baseline(seconds=12.0):
             "sample": f"{done} Saber L=3 MatrixVectorMul instances (Toom-4, {cores} threads, {dt:.1f} s)"}
 
 
+def calib_copy(mib):
+    """A plain copy of known size: read mib MiB, write mib MiB (runtime copy kernel).
+    Its FETCH_SIZE / WRITE_SIZE readings give the scale of those counters."""
+    nbytes = mib << 20
+    src, dst = T.DeviceBuffer(nbytes).zero(), T.DeviceBuffer(nbytes)
+    T.device_synchronize()
+    for _ in range(3):
+        T._hipcheck(T.hip().hipMemcpy(dst.ptr, src.ptr, nbytes, T.D2D), "hipMemcpy")
+    T.device_synchronize()
+    print(json.dumps({"calib_copy_bytes": nbytes, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,12 +106,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run device-to-device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(instances are independent; see DESIGN.md for how B would shard).", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib)
+        return
     print(f"[bench] Saber MatVec B={args.batch} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
     ctx = T.Context(0)
     stream = T.Stream()
