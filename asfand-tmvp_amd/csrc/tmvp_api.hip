@@ -46,7 +46,7 @@ int run(tmvp_ctx *ctx, int backend, const Job &J, int mode, int l, tmvp_stream_t
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     switch (backend) {
     case TMVP_BACKEND_MFMA:
-        if (J.w > 14) return TMVP_EUNSUPPORTED;
+        if (J.w < 8 || J.w > 14) return TMVP_EUNSUPPORTED;
         return hip_status(launch_mfma(J, mode, l, st));
     case TMVP_BACKEND_VALU:
     case TMVP_BACKEND_SCHOOLBOOK:

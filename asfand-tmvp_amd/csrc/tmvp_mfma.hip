@@ -16,8 +16,8 @@
 // (S/src/poly.cu:1155-1205), which is inexact for q = 2^13.
 //
 // Exactness.  The secret is used centred mod 2^w as an int8 (CBD values are
-// at most 5 in magnitude; for w >= 8 that int8 is simply the low byte).  The
-// public coefficient (mod 2^w, w <= 14) is split into limbs a = lo + 128 hi,
+// at most 5 in magnitude; for 8 <= w that int8 is simply the low byte).  The
+// public coefficient (mod 2^w, 8 <= w <= 14) is split into limbs a = lo + 128 hi,
 // lo and hi in [0, 127]; the negated copies stay inside int8.  Each limb
 // product accumulates exactly in int32 (|sum| <= 4*256*127*127 < 2^31) and
 // out = lo + 128 hi is exact mod 2^32.
@@ -37,14 +37,18 @@
 // 16-byte group (the E blocks are stored byte-reversed) so that these windows
 // ascend in memory.
 //
-// Launch: 256 threads = 4 waves, one instance per wave (all rows and terms),
-// every wave with its own LDS slice.
+// Launch: persistent, 256 threads = 4 waves per workgroup.  A wave owns one
+// instance (all rows and terms) at a time in its private LDS slice, walks the
+// batch with a grid stride, and loads the next instance's inputs into
+// registers while the current one is on the matrix cores.
 #include "tmvp_internal.h"
 
 namespace tmvp {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // Per-halfword add of two packed uint16 pairs (wraps mod 2^16).
 __device__ __forceinline__ uint32_t add16x2(uint32_t a, uint32_t b)
@@ -66,64 +70,89 @@ __device__ __forceinline__ uint32_t neg_bytes(uint32_t v)
 }
 __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) { return __builtin_amdgcn_perm(hi, lo, sel); }
 
+// Epilogue on x = exact sum (+ h already added for the rounding epilogues).
 template <int EPI>
 __device__ __forceinline__ uint32_t epi(const Job &J, uint32_t x, uint32_t old, uint32_t aux)
 {
-    if (EPI == kEpiQ2P) return ((x + (uint32_t)J.h) & ((1u << J.eq) - 1u)) >> (J.eq - J.ep);
-    if (EPI == kEpiEnc)
-        return ((x + (uint32_t)J.h - ((aux & 1u) << (J.ep - 1))) & ((1u << J.ep) - 1u)) >> (J.ep - J.et);
-    if (EPI == kEpiDec)
-        return ((x + (uint32_t)J.h - ((aux & ((1u << J.et) - 1u)) << (J.ep - J.et))) & ((1u << J.ep) - 1u)) >>
-               (J.ep - 1);
+    if (EPI == kEpiQ2P) return __builtin_amdgcn_ubfe(x, J.eq - J.ep, J.ep);
+    if (EPI == kEpiEnc) return __builtin_amdgcn_ubfe(x - ((aux & 1u) << (J.ep - 1)), J.ep - J.et, J.et);
+    if (EPI == kEpiDec) return __builtin_amdgcn_ubfe(x - ((aux & ((1u << J.et) - 1u)) << (J.ep - J.et)), J.ep - 1, 1);
     return (x + old) & ((1u << J.w) - 1u);
 }
 
+// Per-wave LDS slice layout (bytes).
+template <int L, int MODE> struct Slice {
+    static constexpr int NR = rows_of(MODE, L);
+    static constexpr int NT = terms_of(MODE, L);
+    static constexpr int NPOLY = NR * NT;      // public polynomials per instance
+    static constexpr int CHUNKS = NPOLY * 32;  // 8-coefficient chunks per instance
+    static constexpr int ITERS = (CHUNKS + 63) / 64;
+    static constexpr int E_BYTES = NPOLY * 2 * kEPoly;
+    static constexpr int BYTES = E_BYTES + NT * 4 * kBand;
+};
+
 template <int L, int MODE, int EPI>
-__global__ __launch_bounds__(256) void mfma_kernel(Job J)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void mfma_kernel(Job J)
 {
-    constexpr int NR = rows_of(MODE, L);
-    constexpr int NT = terms_of(MODE, L);
-    constexpr int NPOLY = NR * NT;             // public polynomials per instance
-    constexpr int CHUNKS = NPOLY * 32;         // 8-coefficient chunks per instance
-    constexpr int E_BYTES = NPOLY * 2 * kEPoly;
-    constexpr int WAVE_BYTES = E_BYTES + NT * 4 * kBand;
-    __shared__ __attribute__((aligned(16))) uint8_t lds[4 * WAVE_BYTES];
+    using S = Slice<L, MODE>;
+    constexpr int NR = S::NR, NT = S::NT, NPOLY = S::NPOLY, CHUNKS = S::CHUNKS, ITERS = S::ITERS;
+    __shared__ __attribute__((aligned(16))) uint8_t lds[4 * S::BYTES];
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t inst = (int64_t)blockIdx.x * 4 + wave;
-    if (inst >= J.batch) return; // wave-uniform; no barrier follows for this wave's partners
-    uint8_t *E = lds + wave * WAVE_BYTES;
-    uint8_t *X = E + E_BYTES;
+    const int64_t stride = (int64_t)gridDim.x * 4;
+    int64_t inst = (int64_t)blockIdx.x * 4 + wave;
+    if (inst >= J.batch) return; // wave-uniform: each wave owns a private LDS slice, no barriers
+    uint8_t *E = lds + wave * S::BYTES;
+    uint8_t *X = E + S::E_BYTES;
     const uint32_t wmask = (1u << J.w) - 1u;
+    const uint32_t lo2 = 0x007f007fu;                 // lo limb of both halfwords (w >= 8)
+    const uint32_t hi2 = (wmask >> 7) * 0x00010001u;  // hi limb mask
+    const uint32_t wm2 = wmask * 0x00010001u;
 
-    // ---- public polynomials -> byte-reversed limb blocks E[e] (+ negated copies)
-    {
-        const uint4 *src = reinterpret_cast<const uint4 *>(J.a + inst * (NPOLY * kN));
-        const uint32_t lo2 = (wmask < 127u ? wmask : 127u) * 0x00010001u;
-        const uint32_t hi2 = (wmask >> 7) * 0x00010001u;
-        const int blk = (lane & 31) >> 2;              // 32-coefficient block of this chunk
-        const int pos = 24 - 8 * (lane & 3);           // byte position inside the reversed block
-        uint4 v[(CHUNKS + 63) / 64];
+    // lane roles
+    const int blk = (lane & 31) >> 2;     // staging: 32-coefficient block of the lane's chunk
+    const int bpos = 24 - 8 * (lane & 3); // staging: byte position inside the reversed block
+    const int il = lane & 15, g = lane >> 4;
+    const int R = lane & 7, limb = (lane >> 3) & 1;
+    const int sh = (il + 1) & 3;
+    // A operand: dword index of the window start in band copy `sh`, without the
+    // compile-time part 16p + 4h (see x0 in the header comment).
+    const int abase_dw = (sh * kBand) / 4 + 8 * (g >> 1) - 4 * (g & 1) + ((il + 17 - sh) >> 2);
+    // B operand: byte offset of E[R - (g>>1) + 8] (minus 64 p per K-chunk) in the lane's limb plane
+    const int bbase = limb * kEPoly + 16 * (1 - (g & 1)) + (R - (g >> 1) + 8) * kEBlock;
+
+    u32x4 va[ITERS];
+    u32x2 vs[NT];
+    auto fetch = [&](int64_t i) {
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + i * (NPOLY * kN));
 #pragma unroll
-        for (int it = 0; it < (CHUNKS + 63) / 64; it++) {
-            const int c = lane + 64 * it;
-            if (CHUNKS % 64 == 0 || c < CHUNKS) v[it] = src[c];
-        }
+        for (int it = 0; it < ITERS; it++)
+            if (CHUNKS % 64 == 0 || lane + 64 * it < CHUNKS) va[it] = __builtin_nontemporal_load(src + lane + 64 * it);
+        const u32x2 *ssrc = reinterpret_cast<const u32x2 *>(J.s + i * (NT * kN));
 #pragma unroll
-        for (int it = 0; it < (CHUNKS + 63) / 64; it++) {
+        for (int j = 0; j < NT; j++) vs[j] = __builtin_nontemporal_load(ssrc + j * 64 + lane);
+    };
+    fetch(inst);
+
+    uint32_t bad = 0;
+    for (;;) {
+        // ---- public polynomials -> byte-reversed limb blocks E[e] (+ negated copies)
+#pragma unroll
+        for (int it = 0; it < ITERS; it++) {
             const int c = lane + 64 * it;
             if (CHUNKS % 64 != 0 && c >= CHUNKS) continue;
-            const int pq = c >> 5;                     // memory order of the polynomial
+            const int pq = c >> 5; // memory order of the polynomial
             int r = 0, j = pq;
             if (MODE == kMatVec) { r = pq / L; j = pq % L; }
             if (MODE == kMatVecT) { r = pq % L; j = pq / L; }
-            const uint32_t l0 = v[it].x & lo2, l1 = v[it].y & lo2, l2 = v[it].z & lo2, l3 = v[it].w & lo2;
-            const uint32_t h0 = (v[it].x >> 7) & hi2, h1 = (v[it].y >> 7) & hi2;
-            const uint32_t h2 = (v[it].z >> 7) & hi2, h3 = (v[it].w >> 7) & hi2;
+            const u32x4 v = va[it];
+            const uint32_t l0 = v.x & lo2, l1 = v.y & lo2, l2 = v.z & lo2, l3 = v.w & lo2;
+            const uint32_t h0 = (v.x >> 7) & hi2, h1 = (v.y >> 7) & hi2;
+            const uint32_t h2 = (v.z >> 7) & hi2, h3 = (v.w >> 7) & hi2;
             // coefficient u of the chunk goes to byte 7 - u
             const uint2 plo = make_uint2(perm(l3, l2, 0x00020406u), perm(l1, l0, 0x00020406u));
             const uint2 phi = make_uint2(perm(h3, h2, 0x00020406u), perm(h1, h0, 0x00020406u));
-            uint8_t *slot = E + (r * NT + j) * 2 * kEPoly + pos;
+            uint8_t *slot = E + (r * NT + j) * 2 * kEPoly + bpos;
             *(uint2 *)(slot + (blk + 8) * kEBlock) = plo;
             *(uint2 *)(slot + kEPoly + (blk + 8) * kEBlock) = phi;
             if (blk > 0) {
@@ -131,37 +160,22 @@ __global__ __launch_bounds__(256) void mfma_kernel(Job J)
                 *(uint2 *)(slot + kEPoly + blk * kEBlock) = make_uint2(neg_small(phi.x), neg_small(phi.y));
             }
         }
-    }
-    // ---- secret polynomials -> int8 band X[x] = ts(x - 32), stored 4x shifted:
-    //      copy c holds X[x + c] at byte x.
-    {
-        const uint2 *src = reinterpret_cast<const uint2 *>(J.s + inst * (NT * kN));
-        uint32_t bad = 0;
+        // ---- secret polynomials -> int8 band X[x] = ts(x - 32), stored 4x shifted:
+        //      copy c holds X[x + c] at byte x.  For w >= 8 the centred int8 is the low byte.
 #pragma unroll
         for (int j = 0; j < NT; j++) {
-            const uint2 sv = src[j * 64 + lane];
-            uint32_t pos;
-            if (J.w >= 8) {
-                pos = perm(sv.y, sv.x, 0x06040200u); // low bytes = int8 of the centred value
-            } else {
-                const int sh = 32 - J.w;
-                const uint32_t cs[4] = {sv.x & 0xffffu, sv.x >> 16, sv.y & 0xffffu, sv.y >> 16};
-                pos = 0;
-#pragma unroll
-                for (int u = 0; u < 4; u++) pos |= ((uint32_t)(((int)(cs[u] << sh)) >> sh) & 0xffu) << (8 * u);
-            }
+            const u32x2 sv = vs[j];
+            const uint32_t pos = perm(sv.y, sv.x, 0x06040200u);
             // range check: centred value in [-127, 127]  <=>  ((s + 127) mod 2^w) + 1 <= 255
-            const uint32_t wm2 = wmask * 0x00010001u;
-            const uint32_t t0 = add16x2(sv.x, 0x007f007fu) & wm2;
-            const uint32_t t1 = add16x2(sv.y, 0x007f007fu) & wm2;
-            bad |= (add16x2(t0, 0x00010001u) | add16x2(t1, 0x00010001u)) & 0xff00ff00u;
+            bad |= (add16x2(add16x2(sv.x, 0x007f007fu) & wm2, 0x00010001u) |
+                    add16x2(add16x2(sv.y, 0x007f007fu) & wm2, 0x00010001u));
             const uint32_t neg = neg_bytes(pos);
-            // dword k of X: k = lane + 8 holds pos of this lane; k = lane - 56 (lanes 56..63) holds neg
+            // dword k of X: k = lane + 8 holds `pos` of this lane; k = lane - 56 (lanes 56..63) holds `neg`
             uint32_t nxt_pos = __shfl_down(pos, 1);
             uint32_t nxt_neg = __shfl_down(neg, 1);
             const uint32_t first_pos = __shfl(pos, 0);
             if (lane == 63) {
-                nxt_pos = 0u;        // X[288..291] (never read as data)
+                nxt_pos = 0u;        // bytes past the band (never read as data)
                 nxt_neg = first_pos; // dword 8 follows dword 7
             }
             uint8_t *band = X + j * 4 * kBand;
@@ -176,81 +190,98 @@ __global__ __launch_bounds__(256) void mfma_kernel(Job J)
                     *(uint32_t *)(band + c * kBand + 4 * lane - 224) = __builtin_amdgcn_alignbyte(nxt_neg, neg, c);
             }
         }
-        if (bad) *J.status = 1;
-    }
-    // The slice is private to this wave: LDS operations of one wave complete in
-    // order, so a wave-level fence is enough before reading it back.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // prefetch the next instance while this one computes
+        const int64_t next = inst + stride;
+        if (next < J.batch) fetch(next);
 
-    const int il = lane & 15, g = lane >> 4;
-    const int R = lane & 7, limb = (lane >> 3) & 1;
-    const int sh = (il + 1) & 3;
+        // The slice is private to this wave and LDS operations of one wave
+        // complete in order: a wave-level fence orders the staging writes
+        // before the reads below.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-    v4i acc[NR][2];
+        // The rounding constant (h1 or h2) enters through the lo-limb accumulators.
+        const int a0 = (EPI != kEpiMod && limb == 0) ? J.h : 0;
+        v4i acc[NR][2];
 #pragma unroll
-    for (int r = 0; r < NR; r++) acc[r][0] = acc[r][1] = v4i{0, 0, 0, 0};
+        for (int r = 0; r < NR; r++) acc[r][0] = acc[r][1] = v4i{a0, a0, a0, a0};
 
 #pragma unroll
-    for (int j = 0; j < NT; j++) {
-        // A operand: Toeplitz tiles of s_j for the four K-chunks and both halves.
-        const uint8_t *copy = X + j * 4 * kBand + sh * kBand;
-        v4i afr[4][2];
+        for (int j = 0; j < NT; j++) {
+            // A operand: Toeplitz tiles of s_j for the four K-chunks and both halves.
+            const uint32_t *aw = reinterpret_cast<const uint32_t *>(X + j * 4 * kBand) + abase_dw;
+            v4i afr[4][2];
 #pragma unroll
-        for (int p = 0; p < 4; p++) {
+            for (int p = 0; p < 4; p++)
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int x0 = 32 * (2 * p + (g >> 1)) + 16 * h - 16 * (g & 1) + il + 17;
-                const uint32_t *w = reinterpret_cast<const uint32_t *>(copy + (x0 - sh));
-                afr[p][h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const uint8_t *slot = E + ((r * NT + j) * 2 + limb) * kEPoly + 16 * (1 - (g & 1));
+                for (int h = 0; h < 2; h++) {
+                    const uint32_t *w = aw + 16 * p + 4 * h;
+                    afr[p][h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+                }
+            // K-chunk outermost: 2*NR independent accumulators between dependent MFMAs.
 #pragma unroll
             for (int p = 0; p < 4; p++) {
-                const int e = R - (2 * p + (g >> 1)) + 8;
-                const v4i bfr = *(const v4i *)(slot + e * kEBlock);
-                acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][0], bfr, acc[r][0], 0, 0, 0);
-                acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][1], bfr, acc[r][1], 0, 0, 0);
+                v4i bfr[NR];
+#pragma unroll
+                for (int r = 0; r < NR; r++)
+                    bfr[r] = *(const v4i *)(E + (r * NT + j) * 2 * kEPoly + bbase - 64 * p);
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][0], bfr[r], acc[r][0], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][1], bfr[r], acc[r][1], 0, 0, 0);
+                }
             }
         }
-    }
 
-    // ---- epilogue: lanes with limb 0 own block column R; rows i = 16h + 4g + v.
-    uint16_t *obase = J.out + inst * (NR * kN);
-    const uint16_t *abase = (EPI == kEpiEnc || EPI == kEpiDec) ? J.aux + inst * kN : nullptr;
+        // ---- epilogue: lanes with limb 0 own block column R; rows i = 16h + 4g + v.
+        uint16_t *obase = J.out + inst * (NR * kN) + 32 * R + 4 * g;
+        const uint16_t *xbase = (EPI == kEpiEnc || EPI == kEpiDec) ? J.aux + inst * kN + 32 * R + 4 * g : nullptr;
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
+        for (int r = 0; r < NR; r++) {
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            uint32_t val[4];
+            for (int h = 0; h < 2; h++) {
+                uint32_t val[4];
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int hiv = __builtin_amdgcn_mov_dpp(acc[r][h][v], 0x128, 0xf, 0xf, false); // row_ror:8
-                val[v] = (uint32_t)acc[r][h][v] + ((uint32_t)hiv << 7);
-            }
-            if (limb == 0) {
-                const int m0 = 32 * R + 16 * h + 4 * g;
-                uint2 oldv = make_uint2(0u, 0u), auxv = make_uint2(0u, 0u);
-                if (EPI == kEpiMod && J.accumulate) oldv = *(const uint2 *)(obase + r * kN + m0);
-                if (EPI == kEpiEnc || EPI == kEpiDec) auxv = *(const uint2 *)(abase + m0);
-                const uint32_t o0 = epi<EPI>(J, val[0], oldv.x & 0xffffu, auxv.x & 0xffffu);
-                const uint32_t o1 = epi<EPI>(J, val[1], oldv.x >> 16, auxv.x >> 16);
-                const uint32_t o2 = epi<EPI>(J, val[2], oldv.y & 0xffffu, auxv.y & 0xffffu);
-                const uint32_t o3 = epi<EPI>(J, val[3], oldv.y >> 16, auxv.y >> 16);
-                *(uint2 *)(obase + r * kN + m0) = make_uint2(o0 | (o1 << 16), o2 | (o3 << 16));
+                for (int v = 0; v < 4; v++) {
+                    const int hiv = __builtin_amdgcn_mov_dpp(acc[r][h][v], 0x128, 0xf, 0xf, false); // row_ror:8
+                    val[v] = (uint32_t)acc[r][h][v] + ((uint32_t)hiv << 7);
+                }
+                if (limb == 0) {
+                    uint2 oldv = make_uint2(0u, 0u), auxv = make_uint2(0u, 0u);
+                    if (EPI == kEpiMod && J.accumulate) oldv = *(const uint2 *)(obase + r * kN + 16 * h);
+                    if (EPI == kEpiEnc || EPI == kEpiDec) auxv = *(const uint2 *)(xbase + 16 * h);
+                    const uint32_t o0 = epi<EPI>(J, val[0], oldv.x & 0xffffu, auxv.x & 0xffffu);
+                    const uint32_t o1 = epi<EPI>(J, val[1], oldv.x >> 16, auxv.x >> 16);
+                    const uint32_t o2 = epi<EPI>(J, val[2], oldv.y & 0xffffu, auxv.y & 0xffffu);
+                    const uint32_t o3 = epi<EPI>(J, val[3], oldv.y >> 16, auxv.y >> 16);
+                    __builtin_nontemporal_store(u32x2{o0 | (o1 << 16), o2 | (o3 << 16)},
+                                                (u32x2 *)(obase + r * kN + 16 * h));
+                }
             }
         }
+        inst = next;
+        if (inst >= J.batch) break;
     }
+    if (bad & 0xff00ff00u) *J.status = 1;
 }
 
+// Persistent launch: enough workgroups to fill every CU at the occupancy the
+// LDS slice allows, each wave looping over instances with a grid stride.
 template <int L, int MODE, int EPI>
 static hipError_t go(const Job &J, hipStream_t stream)
 {
-    const int64_t blocks = (J.batch + 3) / 4;
+    static int resident = 0; // workgroups that fit on the device at once
+    if (resident == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mfma_kernel<L, MODE, EPI>, 256, 0);
+        if (e != hipSuccess) return e;
+        resident = cus * (per_cu > 0 ? per_cu : 1);
+    }
+    const int64_t needed = (J.batch + 3) / 4;
+    const int64_t blocks = needed < resident ? needed : resident;
     hipLaunchKernelGGL((mfma_kernel<L, MODE, EPI>), dim3((unsigned)blocks), dim3(256), 0, stream, J);
     return hipGetLastError();
 }

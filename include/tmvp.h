@@ -105,7 +105,7 @@ int tmvp_version(void);
  * submatrix_cuda -> matvecp_cuda -> matvecout_cuda chain
  * (S/src/poly.cu:1116-1129, :1295-1392) and the per-(i,j) triangle loops of
  * MatVecMul_gpu_shared (S/src/poly.cu:281-284).  eq_bits in [1,16] (VALU) or
- * [1,14] (MFMA). */
+ * [8,14] (MFMA). */
 int tmvp_polymul(tmvp_ctx *ctx, int backend, uint16_t *d_c, const uint16_t *d_a,
                  const uint16_t *d_s, int64_t batch, int eq_bits, int flags,
                  tmvp_stream_t stream);

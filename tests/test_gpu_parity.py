@@ -210,6 +210,7 @@ def test_argument_errors(ctx):
     lib = T.load_library()
     buf = T.DeviceBuffer(4096)
     assert lib.tmvp_polymul(ctx.handle, T.MFMA, buf.ptr, buf.ptr, buf.ptr, 1, 15, 0, None) == -4
+    assert lib.tmvp_polymul(ctx.handle, T.MFMA, buf.ptr, buf.ptr, buf.ptr, 1, 7, 0, None) == -4
     assert lib.tmvp_polymul(ctx.handle, 7, buf.ptr, buf.ptr, buf.ptr, 1, 13, 0, None) == -1
     assert lib.tmvp_polymul(ctx.handle, T.MFMA, buf.ptr, buf.ptr, buf.ptr, 1, 17, 0, None) == -1
     assert lib.tmvp_matvec(ctx.handle, 9, T.MFMA, 0, buf.ptr, buf.ptr, buf.ptr, 1, 0, 0, None) == -1
