@@ -31,16 +31,19 @@
 // aligned 16-byte LDS read per MFMA pair (it feeds both halves h).
 //
 // An A-operand lane needs 16 consecutive band bytes at an offset whose
-// misalignment (i + 1) mod 4 is fixed per lane.  The band is therefore stored
-// four times, shifted by 0..3 bytes, and every lane reads its own copy with
-// dword-aligned LDS loads.  K indices run in reversed jj order inside each
+// misalignment (i + 1) mod 8 is fixed per lane.  The band is therefore stored
+// eight times, shifted by 0..7 bytes, and every lane reads its own copy with
+// one 8-byte-aligned two-address LDS load (unaligned 16-byte LDS loads are
+// exact on gfx950 but many times slower).  K indices run in reversed jj order inside each
 // 16-byte group (the E blocks are stored byte-reversed) so that these windows
 // ascend in memory.
 //
 // Launch: persistent, 256 threads = 4 waves per workgroup.  A wave owns one
-// instance (all rows and terms) at a time in its private LDS slice, walks the
-// batch with a grid stride, and loads the next instance's inputs into
-// registers while the current one is on the matrix cores.
+// instance at a time in a private LDS slice, stages it one secret term at a
+// time, walks the batch with a grid stride, and loads each term of the next
+// instance into registers as soon as the current one has been staged.
+#include <stdlib.h>
+
 #include "tmvp_internal.h"
 
 namespace tmvp {
@@ -50,6 +53,11 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// Per-halfword logical right shift of a packed uint16 pair.
+__device__ __forceinline__ uint32_t shr16x2(uint32_t a, int n)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) >> (unsigned short)n);
+}
 // Per-halfword add of two packed uint16 pairs (wraps mod 2^16).
 __device__ __forceinline__ uint32_t add16x2(uint32_t a, uint32_t b)
 {
@@ -80,33 +88,47 @@ __device__ __forceinline__ uint32_t epi(const Job &J, uint32_t x, uint32_t old, 
     return (x + old) & ((1u << J.w) - 1u);
 }
 
-// Per-wave LDS slice layout (bytes).
+// Per-wave LDS slice: the E blocks of every row for ONE secret term j, and the
+// 4 shifted copies of that term's band.  Terms are staged one after another,
+// so the staging of term j+1 can overlap the matrix-core work of term j.
 template <int L, int MODE> struct Slice {
     static constexpr int NR = rows_of(MODE, L);
     static constexpr int NT = terms_of(MODE, L);
-    static constexpr int NPOLY = NR * NT;      // public polynomials per instance
-    static constexpr int CHUNKS = NPOLY * 32;  // 8-coefficient chunks per instance
-    static constexpr int ITERS = (CHUNKS + 63) / 64;
-    static constexpr int E_BYTES = NPOLY * 2 * kEPoly;
-    static constexpr int BYTES = E_BYTES + NT * 4 * kBand;
+    static constexpr int NPOLY = NR * NT;       // public polynomials per instance
+    static constexpr int CJ = NR * 32;          // 8-coefficient chunks of one term
+    static constexpr int ITJ = (CJ + 63) / 64;  // staging loads per lane per term
+    static constexpr int E_BYTES = NR * 2 * kEPoly;
+    static constexpr int BYTES = E_BYTES + 4 * kBand;
 };
 
+// DPP wave_shl:1 - lane i receives lane i + 1 (lane 63 receives 0).
+__device__ __forceinline__ uint32_t from_next_lane(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
+}
+
+// Register budget: 3 waves per SIMD (168 VGPRs) up to three rows and terms,
+// 2 waves per SIMD for the four-row FireSaber / Sable-3 shapes.
+template <int L, int MODE> constexpr int kWavesPerEU = (rows_of(MODE, L) * terms_of(MODE, L) > 9) ? 2 : 3;
+
 template <int L, int MODE, int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void mfma_kernel(Job J)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<L, MODE>))) void mfma_kernel(Job J)
 {
     using S = Slice<L, MODE>;
-    constexpr int NR = S::NR, NT = S::NT, NPOLY = S::NPOLY, CHUNKS = S::CHUNKS, ITERS = S::ITERS;
-    __shared__ __attribute__((aligned(16))) uint8_t lds[4 * S::BYTES];
+    constexpr int NR = S::NR, NT = S::NT, NPOLY = S::NPOLY, CJ = S::CJ, ITJ = S::ITJ;
+    // Two slices per wave: term t is staged into slice t & 1 while the term
+    // before it is read from the other one.
+    __shared__ __attribute__((aligned(16))) uint8_t lds[4 * 2 * S::BYTES];
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t stride = (int64_t)gridDim.x * 4;
     int64_t inst = (int64_t)blockIdx.x * 4 + wave;
-    if (inst >= J.batch) return; // wave-uniform: each wave owns a private LDS slice, no barriers
-    uint8_t *E = lds + wave * S::BYTES;
-    uint8_t *X = E + S::E_BYTES;
+    if (inst >= J.batch) return; // wave-uniform: each wave owns private LDS slices, no barriers
+    uint8_t *const slice0 = lds + wave * 2 * S::BYTES;
+    int buf = 0; // slice holding the term about to be read
     const uint32_t wmask = (1u << J.w) - 1u;
-    const uint32_t lo2 = 0x007f007fu;                 // lo limb of both halfwords (w >= 8)
-    const uint32_t hi2 = (wmask >> 7) * 0x00010001u;  // hi limb mask
+    const uint32_t lo4 = 0x7f7f7f7fu;                 // lo limb mask of four gathered bytes
+    const uint32_t hi4 = (wmask >> 7) * 0x01010101u;  // hi limb mask of four gathered bytes (w >= 8)
     const uint32_t wm2 = wmask * 0x00010001u;
 
     // lane roles
@@ -115,44 +137,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
     const int il = lane & 15, g = lane >> 4;
     const int R = lane & 7, limb = (lane >> 3) & 1;
     const int sh = (il + 1) & 3;
-    // A operand: dword index of the window start in band copy `sh`, without the
-    // compile-time part 16p + 4h (see x0 in the header comment).
-    const int abase_dw = (sh * kBand) / 4 + 8 * (g >> 1) - 4 * (g & 1) + ((il + 17 - sh) >> 2);
+    // A operand: dword index of the window start x0 - sh in band copy `sh`, without
+    // the compile-time part 16p + 4h (x0 = 32D + 16h - 16(g&1) + il + 17, D = 2p + (g>>1);
+    // il + 17 - sh is a multiple of 4).
+    const int abase = (sh * kBand) / 4 + 8 * (g >> 1) - 4 * (g & 1) + ((il + 17 - sh) >> 2);
     // B operand: byte offset of E[R - (g>>1) + 8] (minus 64 p per K-chunk) in the lane's limb plane
     const int bbase = limb * kEPoly + 16 * (1 - (g & 1)) + (R - (g >> 1) + 8) * kEBlock;
 
-    u32x4 va[ITERS];
+    // Registers holding one instance's inputs, term by term.
+    u32x4 va[NT][ITJ];
     u32x2 vs[NT];
-    auto fetch = [&](int64_t i) {
+    auto fetch_term = [&](int64_t i, int j) {
         const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + i * (NPOLY * kN));
 #pragma unroll
-        for (int it = 0; it < ITERS; it++)
-            if (CHUNKS % 64 == 0 || lane + 64 * it < CHUNKS) va[it] = __builtin_nontemporal_load(src + lane + 64 * it);
-        const u32x2 *ssrc = reinterpret_cast<const u32x2 *>(J.s + i * (NT * kN));
-#pragma unroll
-        for (int j = 0; j < NT; j++) vs[j] = __builtin_nontemporal_load(ssrc + j * 64 + lane);
+        for (int it = 0; it < ITJ; it++) {
+            const int u = lane + 64 * it; // chunk of this term: row r = u >> 5, piece u & 31
+            if (CJ % 64 == 0 || u < CJ) {
+                const int r = u >> 5;
+                const int pq = (MODE == kMatVec) ? r * L + j : (MODE == kMatVecT) ? j * L + r : j;
+                va[j][it] = src[pq * 32 + (u & 31)];
+            }
+        }
+        vs[j] = reinterpret_cast<const u32x2 *>(J.s + i * (NT * kN))[j * 64 + lane];
     };
-    fetch(inst);
+#pragma unroll
+    for (int j = 0; j < NT; j++) fetch_term(inst, j);
 
     uint32_t bad = 0;
-    for (;;) {
-        // ---- public polynomials -> byte-reversed limb blocks E[e] (+ negated copies)
+    // Stage term j from va[j] / vs[j] into slice `into`.
+    auto stage = [&](int j, int into) {
+        uint8_t *const E = slice0 + into * S::BYTES;
+        uint8_t *const X = E + S::E_BYTES;
+        // ---- public polynomials of term j -> byte-reversed limb blocks E[r][limb][e]
 #pragma unroll
-        for (int it = 0; it < ITERS; it++) {
-            const int c = lane + 64 * it;
-            if (CHUNKS % 64 != 0 && c >= CHUNKS) continue;
-            const int pq = c >> 5; // memory order of the polynomial
-            int r = 0, j = pq;
-            if (MODE == kMatVec) { r = pq / L; j = pq % L; }
-            if (MODE == kMatVecT) { r = pq % L; j = pq / L; }
-            const u32x4 v = va[it];
-            const uint32_t l0 = v.x & lo2, l1 = v.y & lo2, l2 = v.z & lo2, l3 = v.w & lo2;
-            const uint32_t h0 = (v.x >> 7) & hi2, h1 = (v.y >> 7) & hi2;
-            const uint32_t h2 = (v.z >> 7) & hi2, h3 = (v.w >> 7) & hi2;
-            // coefficient u of the chunk goes to byte 7 - u
-            const uint2 plo = make_uint2(perm(l3, l2, 0x00020406u), perm(l1, l0, 0x00020406u));
-            const uint2 phi = make_uint2(perm(h3, h2, 0x00020406u), perm(h1, h0, 0x00020406u));
-            uint8_t *slot = E + (r * NT + j) * 2 * kEPoly + bpos;
+        for (int it = 0; it < ITJ; it++) {
+            const int u = lane + 64 * it;
+            if (CJ % 64 != 0 && u >= CJ) continue;
+            const int r = u >> 5;
+            const u32x4 v = va[j][it];
+            // hi limb = coefficient >> 7, per 16-bit half
+            const uint32_t s0 = shr16x2(v.x, 7), s1 = shr16x2(v.y, 7), s2 = shr16x2(v.z, 7), s3 = shr16x2(v.w, 7);
+            // gather the low byte of every coefficient, coefficient u of the chunk to byte 7 - u
+            const uint2 plo = make_uint2(perm(v.w, v.z, 0x00020406u) & lo4, perm(v.y, v.x, 0x00020406u) & lo4);
+            const uint2 phi = make_uint2(perm(s3, s2, 0x00020406u) & hi4, perm(s1, s0, 0x00020406u) & hi4);
+            uint8_t *slot = E + r * 2 * kEPoly + bpos;
             *(uint2 *)(slot + (blk + 8) * kEBlock) = plo;
             *(uint2 *)(slot + kEPoly + (blk + 8) * kEBlock) = phi;
             if (blk > 0) {
@@ -160,57 +188,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
                 *(uint2 *)(slot + kEPoly + blk * kEBlock) = make_uint2(neg_small(phi.x), neg_small(phi.y));
             }
         }
-        // ---- secret polynomials -> int8 band X[x] = ts(x - 32), stored 4x shifted:
-        //      copy c holds X[x + c] at byte x.  For w >= 8 the centred int8 is the low byte.
+        // ---- secret s_j -> int8 band X[x] = ts(x - 32), stored 4x shifted: copy c
+        //      holds X[x + c] at byte x.  For w >= 8 the centred int8 is the low byte.
+        const u32x2 sv = vs[j];
+        const uint32_t pos = perm(sv.y, sv.x, 0x06040200u);
+        // range check: centred value in [-127, 127]  <=>  ((s + 127) mod 2^w) + 1 <= 255
+        bad |= (add16x2(add16x2(sv.x, 0x007f007fu) & wm2, 0x00010001u) |
+                add16x2(add16x2(sv.y, 0x007f007fu) & wm2, 0x00010001u));
+        const uint32_t neg = neg_bytes(pos);
+        // Band dword k: k = lane + 8 is `pos` of this lane, k = lane - 56 (lanes
+        // 56..63) is `neg`; dword 72 is zero padding.
+        const uint32_t p1 = from_next_lane(pos);
+        uint32_t n1 = from_next_lane(neg);
+        if (lane == 63) n1 = __builtin_amdgcn_readlane(pos, 0); // dword 8 follows dword 7
+        uint8_t *band = X + 32 + 4 * lane;
+        *(uint32_t *)band = pos;
 #pragma unroll
-        for (int j = 0; j < NT; j++) {
-            const u32x2 sv = vs[j];
-            const uint32_t pos = perm(sv.y, sv.x, 0x06040200u);
-            // range check: centred value in [-127, 127]  <=>  ((s + 127) mod 2^w) + 1 <= 255
-            bad |= (add16x2(add16x2(sv.x, 0x007f007fu) & wm2, 0x00010001u) |
-                    add16x2(add16x2(sv.y, 0x007f007fu) & wm2, 0x00010001u));
-            const uint32_t neg = neg_bytes(pos);
-            // dword k of X: k = lane + 8 holds `pos` of this lane; k = lane - 56 (lanes 56..63) holds `neg`
-            uint32_t nxt_pos = __shfl_down(pos, 1);
-            uint32_t nxt_neg = __shfl_down(neg, 1);
-            const uint32_t first_pos = __shfl(pos, 0);
-            if (lane == 63) {
-                nxt_pos = 0u;        // bytes past the band (never read as data)
-                nxt_neg = first_pos; // dword 8 follows dword 7
-            }
-            uint8_t *band = X + j * 4 * kBand;
-            *(uint32_t *)(band + 32 + 4 * lane) = pos;
+        for (int c = 1; c < 4; c++) *(uint32_t *)(band + c * kBand) = __builtin_amdgcn_alignbyte(p1, pos, c);
+        if (lane >= 56) {
+            uint8_t *nb = X + 4 * lane - 224;
+            *(uint32_t *)nb = neg;
 #pragma unroll
-            for (int c = 1; c < 4; c++)
-                *(uint32_t *)(band + c * kBand + 32 + 4 * lane) = __builtin_amdgcn_alignbyte(nxt_pos, pos, c);
-            if (lane >= 56) {
-                *(uint32_t *)(band + 4 * lane - 224) = neg;
-#pragma unroll
-                for (int c = 1; c < 4; c++)
-                    *(uint32_t *)(band + c * kBand + 4 * lane - 224) = __builtin_amdgcn_alignbyte(nxt_neg, neg, c);
-            }
+            for (int c = 1; c < 4; c++) *(uint32_t *)(nb + c * kBand) = __builtin_amdgcn_alignbyte(n1, neg, c);
         }
-        // prefetch the next instance while this one computes
-        const int64_t next = inst + stride;
-        if (next < J.batch) fetch(next);
-
-        // The slice is private to this wave and LDS operations of one wave
-        // complete in order: a wave-level fence orders the staging writes
-        // before the reads below.
+    };
+    // The slices are private to this wave and the LDS completes one wave's
+    // operations in order: a wave-level fence at the end of every term keeps
+    // the compiler from moving a term's staging writes past its reads, or the
+    // reads of a slice past the staging that later overwrites it.
+    auto wave_fence = [] {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
 
-        // The rounding constant (h1 or h2) enters through the lo-limb accumulators.
-        const int a0 = (EPI != kEpiMod && limb == 0) ? J.h : 0;
+    // Software pipeline: the staging of the following term (the next term of
+    // this instance, or term 0 of the next instance) goes to the other slice
+    // and is issued between the MFMAs of the current term.
+    stage(0, 0);
+    if (inst + stride < J.batch) fetch_term(inst + stride, 0);
+    wave_fence();
+    for (;;) {
+        const int64_t next = inst + stride;
         v4i acc[NR][2];
 #pragma unroll
-        for (int r = 0; r < NR; r++) acc[r][0] = acc[r][1] = v4i{a0, a0, a0, a0};
-
-#pragma unroll
         for (int j = 0; j < NT; j++) {
-            // A operand: Toeplitz tiles of s_j for the four K-chunks and both halves.
-            const uint32_t *aw = reinterpret_cast<const uint32_t *>(X + j * 4 * kBand) + abase_dw;
+            const uint8_t *const E = slice0 + buf * S::BYTES;
+            const uint8_t *const X = E + S::E_BYTES;
+            // A operand: Toeplitz tiles of s_j for the four K-chunks and both halves,
+            // 16 bytes from a dword-aligned window each.
+            const uint32_t *aw = reinterpret_cast<const uint32_t *>(X) + abase;
             v4i afr[4][2];
 #pragma unroll
             for (int p = 0; p < 4; p++)
@@ -219,33 +246,70 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
                     const uint32_t *w = aw + 16 * p + 4 * h;
                     afr[p][h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
                 }
+            // B operand of K-chunk 0; chunk p + 1 is read while chunk p is multiplied.
+            v4i bcur[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) bcur[r] = *(const v4i *)(E + r * 2 * kEPoly + bbase);
+
+            if (j + 1 < NT) {
+                stage(j + 1, buf ^ 1);
+                if (next < J.batch) fetch_term(next, j + 1);
+            } else if (next < J.batch) {
+                stage(0, buf ^ 1); // va[0] / vs[0] already hold the next instance's term 0
+                if (next + stride < J.batch) fetch_term(next + stride, 0);
+            }
+
             // K-chunk outermost: 2*NR independent accumulators between dependent MFMAs.
 #pragma unroll
             for (int p = 0; p < 4; p++) {
-                v4i bfr[NR];
+                v4i bnxt[NR];
+                if (p < 3) {
 #pragma unroll
-                for (int r = 0; r < NR; r++)
-                    bfr[r] = *(const v4i *)(E + (r * NT + j) * 2 * kEPoly + bbase - 64 * p);
+                    for (int r = 0; r < NR; r++) bnxt[r] = *(const v4i *)(E + r * 2 * kEPoly + bbase - 64 * (p + 1));
+                }
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
-                    acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][0], bfr[r], acc[r][0], 0, 0, 0);
-                    acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][1], bfr[r], acc[r][1], 0, 0, 0);
+                    const v4i zero = {0, 0, 0, 0};
+                    const bool first = (j == 0 && p == 0);
+                    acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][0], bcur[r], first ? zero : acc[r][0], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][1], bcur[r], first ? zero : acc[r][1], 0, 0, 0);
+                }
+                if (p < 3) {
+#pragma unroll
+                    for (int r = 0; r < NR; r++) bcur[r] = bnxt[r];
                 }
             }
+            // Schedule: fragment reads for chunk 0 first, then per chunk the next
+            // chunk's B reads and one MFMA per three staging VALU instructions.
+            __builtin_amdgcn_sched_group_barrier(0x100, 16 + NR, 0);
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                if (p < 3) __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
+#pragma unroll
+                for (int k = 0; k < 2 * NR; k++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+                }
+            }
+            buf ^= 1;
+            wave_fence();
         }
 
         // ---- epilogue: lanes with limb 0 own block column R; rows i = 16h + 4g + v.
+        const uint32_t hadd = (EPI != kEpiMod) ? (uint32_t)J.h : 0u; // h1 or h2 of the rounding
         uint16_t *obase = J.out + inst * (NR * kN) + 32 * R + 4 * g;
         const uint16_t *xbase = (EPI == kEpiEnc || EPI == kEpiDec) ? J.aux + inst * kN + 32 * R + 4 * g : nullptr;
 #pragma unroll
         for (int r = 0; r < NR; r++) {
 #pragma unroll
             for (int h = 0; h < 2; h++) {
+                // hi-limb lanes contribute 128 * hi (+ the rounding constant); row_ror:8
+                // brings them next to their lo-limb lanes.
                 uint32_t val[4];
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int hiv = __builtin_amdgcn_mov_dpp(acc[r][h][v], 0x128, 0xf, 0xf, false); // row_ror:8
-                    val[v] = (uint32_t)acc[r][h][v] + ((uint32_t)hiv << 7);
+                    const uint32_t scaled = ((uint32_t)acc[r][h][v] << 7) + hadd;
+                    val[v] = (uint32_t)acc[r][h][v] + (uint32_t)__builtin_amdgcn_mov_dpp((int)scaled, 0x128, 0xf, 0xf, false);
                 }
                 if (limb == 0) {
                     uint2 oldv = make_uint2(0u, 0u), auxv = make_uint2(0u, 0u);
@@ -255,8 +319,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void m
                     const uint32_t o1 = epi<EPI>(J, val[1], oldv.x >> 16, auxv.x >> 16);
                     const uint32_t o2 = epi<EPI>(J, val[2], oldv.y & 0xffffu, auxv.y & 0xffffu);
                     const uint32_t o3 = epi<EPI>(J, val[3], oldv.y >> 16, auxv.y >> 16);
-                    __builtin_nontemporal_store(u32x2{o0 | (o1 << 16), o2 | (o3 << 16)},
-                                                (u32x2 *)(obase + r * kN + 16 * h));
+                    *(u32x2 *)(obase + r * kN + 16 * h) = u32x2{o0 | (o1 << 16), o2 | (o3 << 16)};
                 }
             }
         }
@@ -278,7 +341,13 @@ static hipError_t go(const Job &J, hipStream_t stream)
         if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mfma_kernel<L, MODE, EPI>, 256, 0);
         if (e != hipSuccess) return e;
-        resident = cus * (per_cu > 0 ? per_cu : 1);
+        if (per_cu < 1) per_cu = 1;
+        // Tuning override: workgroups per CU, capped by what fits.
+        if (const char *env = getenv("TMVP_WGS_PER_CU")) {
+            const int want = atoi(env);
+            if (want >= 1 && want < per_cu) per_cu = want;
+        }
+        resident = cus * per_cu;
     }
     const int64_t needed = (J.batch + 3) / 4;
     const int64_t blocks = needed < resident ? needed : resident;
