@@ -35,6 +35,19 @@ BATCH = 65536
 BYTES_PER_INSTANCE = 9 * N * 2 + 3 * N * 2 + 3 * N * 2
 MACS_PER_INSTANCE = 9 * N * N
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "mfma_matvec_saber_b65536_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the MFMA kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration)."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def synthetic_inputs(batch, seed=1):
@@ -170,7 +183,8 @@ def main():
         "mfma_over_valu": value / valu_value,
         "polymul_per_s": value * 9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if args.batch == BATCH else None},
         "mfma_tops_useful": MACS_PER_INSTANCE * 2 * value / 1e12,
         "outputs_equal": same,
     }
