@@ -6,12 +6,24 @@
 
 #include "../../include/tmvp.h"
 #include "tmvp_internal.h"
+#include "tmvp_kem.h"
 
 using namespace tmvp;
+
+namespace {
+constexpr int kMaxWorkspaces = 16;
+struct Workspace {
+    hipStream_t stream;
+    void *ptr;
+    size_t bytes;
+};
+} // namespace
 
 struct tmvp_ctx {
     int device;
     int *d_status; // device word set by MFMA launches on out-of-range secrets
+    Workspace ws[kMaxWorkspaces];
+    int nws;
 };
 
 namespace {
@@ -54,6 +66,69 @@ int run(tmvp_ctx *ctx, int backend, const Job &J, int mode, int l, tmvp_stream_t
     default:
         return TMVP_EARG;
     }
+}
+
+KemParams kem_params(int ps)
+{
+    const PS &p = kParams[ps];
+    KemParams P;
+    P.l = p.l;
+    P.eq = p.eq;
+    P.ep = p.ep;
+    P.et = p.et;
+    P.mu = p.mu;
+    P.sable = p.sable;
+    P.h1 = h1_of(p);
+    P.h2 = h2_of(p);
+    P.pvc = p.l * p.ep * 32;
+    P.indcpa_sk = p.sable ? p.l * 64 : p.l * p.eq * 32;
+    P.pk = P.pvc + 32;
+    P.sk = P.indcpa_sk + P.pk + 64;
+    P.ct = P.pvc + p.et * 32;
+    return P;
+}
+
+// Workspace of at least `bytes` for `stream`; growing it synchronises the device.
+int workspace(tmvp_ctx *ctx, hipStream_t stream, size_t bytes, uint8_t **out)
+{
+    Workspace *w = nullptr;
+    for (int i = 0; i < ctx->nws; i++)
+        if (ctx->ws[i].stream == stream) w = &ctx->ws[i];
+    if (!w) {
+        if (ctx->nws == kMaxWorkspaces) return TMVP_ENOMEM;
+        w = &ctx->ws[ctx->nws++];
+        w->stream = stream;
+        w->ptr = nullptr;
+        w->bytes = 0;
+    }
+    if (w->bytes < bytes) {
+        if (hipDeviceSynchronize() != hipSuccess) return TMVP_EHIP;
+        if (w->ptr) (void)hipFree(w->ptr);
+        w->ptr = nullptr;
+        w->bytes = 0;
+        if (hipMalloc(&w->ptr, bytes) != hipSuccess) return TMVP_ENOMEM;
+        w->bytes = bytes;
+    }
+    *out = static_cast<uint8_t *>(w->ptr);
+    return TMVP_OK;
+}
+
+bool kem_args_ok(const tmvp_ctx *ctx, int ps, int backend, int64_t batch)
+{
+    return ctx && ps >= 0 && ps < TMVP_NUM_PARAM_SETS && batch >= 0 &&
+           (backend == TMVP_BACKEND_MFMA || backend == TMVP_BACKEND_VALU || backend == TMVP_BACKEND_SCHOOLBOOK);
+}
+
+int kem_setup(tmvp_ctx *ctx, int ps, int64_t batch, tmvp_stream_t stream, KemParams &P, KemWorkspace &W)
+{
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    P = kem_params(ps);
+    uint8_t *base = nullptr;
+    rc = workspace(ctx, reinterpret_cast<hipStream_t>(stream), kem_workspace_bytes(P, batch), &base);
+    if (rc) return rc;
+    kem_workspace_carve(P, batch, base, W);
+    return TMVP_OK;
 }
 
 Job base_job(tmvp_ctx *ctx, uint16_t *out, const uint16_t *a, const uint16_t *s, int64_t batch)
@@ -124,6 +199,7 @@ int tmvp_init(tmvp_ctx **ctx, int device)
     if (!c) return TMVP_ENOMEM;
     c->device = device;
     c->d_status = nullptr;
+    c->nws = 0;
     if (hipMalloc(&c->d_status, sizeof(int)) != hipSuccess) {
         delete c;
         return TMVP_ENOMEM;
@@ -141,6 +217,9 @@ int tmvp_destroy(tmvp_ctx *ctx)
 {
     if (!ctx) return TMVP_EARG;
     int rc = set_device(ctx);
+    if (hipDeviceSynchronize() != hipSuccess) rc = TMVP_EHIP;
+    for (int i = 0; i < ctx->nws; i++)
+        if (ctx->ws[i].ptr) (void)hipFree(ctx->ws[i].ptr);
     if (ctx->d_status) (void)hipFree(ctx->d_status);
     delete ctx;
     return rc;
@@ -222,6 +301,48 @@ int tmvp_innerprod(tmvp_ctx *ctx, int ps, int backend, int epilogue, uint16_t *d
         J.aux = d_aux;
     }
     return run(ctx, backend, J, kInner, p.l, stream);
+}
+
+int tmvp_kem_keypair(tmvp_ctx *ctx, int ps, int backend, uint8_t *d_pk, uint8_t *d_sk, const uint8_t *d_seeds,
+                     int64_t batch, tmvp_stream_t stream)
+{
+    if (!kem_args_ok(ctx, ps, backend, batch)) return TMVP_EARG;
+    if (batch == 0) return TMVP_OK;
+    if (!d_pk || !d_sk || !d_seeds) return TMVP_EARG;
+    KemParams P;
+    KemWorkspace W;
+    int rc = kem_setup(ctx, ps, batch, stream, P, W);
+    if (rc) return rc;
+    return hip_status(kem_keypair(P, W, backend == TMVP_BACKEND_MFMA ? 0 : 1, d_seeds, d_pk, d_sk, batch,
+                                  ctx->d_status, reinterpret_cast<hipStream_t>(stream)));
+}
+
+int tmvp_kem_encaps(tmvp_ctx *ctx, int ps, int backend, uint8_t *d_ct, uint8_t *d_ss, const uint8_t *d_pk,
+                    const uint8_t *d_rnd, int64_t batch, tmvp_stream_t stream)
+{
+    if (!kem_args_ok(ctx, ps, backend, batch)) return TMVP_EARG;
+    if (batch == 0) return TMVP_OK;
+    if (!d_ct || !d_ss || !d_pk || !d_rnd) return TMVP_EARG;
+    KemParams P;
+    KemWorkspace W;
+    int rc = kem_setup(ctx, ps, batch, stream, P, W);
+    if (rc) return rc;
+    return hip_status(kem_encaps(P, W, backend == TMVP_BACKEND_MFMA ? 0 : 1, d_pk, d_rnd, d_ct, d_ss, batch,
+                                 ctx->d_status, reinterpret_cast<hipStream_t>(stream)));
+}
+
+int tmvp_kem_decaps(tmvp_ctx *ctx, int ps, int backend, uint8_t *d_ss, const uint8_t *d_ct, const uint8_t *d_sk,
+                    uint8_t *d_fail, int64_t batch, tmvp_stream_t stream)
+{
+    if (!kem_args_ok(ctx, ps, backend, batch)) return TMVP_EARG;
+    if (batch == 0) return TMVP_OK;
+    if (!d_ss || !d_ct || !d_sk) return TMVP_EARG;
+    KemParams P;
+    KemWorkspace W;
+    int rc = kem_setup(ctx, ps, batch, stream, P, W);
+    if (rc) return rc;
+    return hip_status(kem_decaps(P, W, backend == TMVP_BACKEND_MFMA ? 0 : 1, d_ct, d_sk, d_ss, d_fail, batch,
+                                 ctx->d_status, reinterpret_cast<hipStream_t>(stream)));
 }
 
 } // extern "C"

@@ -1,0 +1,38 @@
+// Batched KEM chains (key generation, encapsulation, decapsulation) on the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace tmvp {
+
+// Parameter set with the derived byte sizes (S/include/params.h:26-49,
+// SB/include/SABLE_params.h:26-72).
+struct KemParams {
+    int l, eq, ep, et, mu, sable, h1, h2;
+    int pvc;       // L * EP * 32: compressed polynomial vector
+    int indcpa_sk; // Saber: L * 13 * 32, Sable: L * 2 * 32
+    int pk, sk, ct;
+};
+
+// Scratch for one batched call (carved from one device allocation).
+struct KemWorkspace {
+    uint16_t *A, *s, *v1, *v2, *bits, *poly;
+    uint8_t *buf, *kr, *ctp;
+};
+
+size_t kem_workspace_bytes(const KemParams &P, int64_t B);
+void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorkspace &W);
+
+// backend: 0 = MFMA, otherwise VALU.  seeds: [B][96] = seed_A | seed_s | z.
+hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *seeds, uint8_t *pk,
+                       uint8_t *sk, int64_t B, int *status, hipStream_t st);
+// rnd: [B][32] randomness per instance.
+hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *pk, const uint8_t *rnd,
+                      uint8_t *ct, uint8_t *ss, int64_t B, int *status, hipStream_t st);
+// fail (optional): [B] bytes, 1 where the ciphertext was rejected.
+hipError_t kem_decaps(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *ct, const uint8_t *sk,
+                      uint8_t *ss, uint8_t *fail, int64_t B, int *status, hipStream_t st);
+
+} // namespace tmvp

@@ -63,7 +63,8 @@ _lib = None
 
 # The exported C-ABI symbols, as declared in include/tmvp.h.
 EXPORTS = ("tmvp_init", "tmvp_destroy", "tmvp_ctx_status", "tmvp_strerror", "tmvp_get_params",
-           "tmvp_version", "tmvp_polymul", "tmvp_matvec", "tmvp_innerprod")
+           "tmvp_version", "tmvp_polymul", "tmvp_matvec", "tmvp_innerprod",
+           "tmvp_kem_keypair", "tmvp_kem_encaps", "tmvp_kem_decaps")
 
 
 def load_library(path=LIB_PATH):
@@ -85,6 +86,9 @@ def load_library(path=LIB_PATH):
         "tmvp_polymul": (i, [P, i, P, P, P, i64, i, i, P]),
         "tmvp_matvec": (i, [P, i, i, i, P, P, P, i64, i, i, P]),
         "tmvp_innerprod": (i, [P, i, i, i, P, P, P, P, i64, i, P]),
+        "tmvp_kem_keypair": (i, [P, i, i, P, P, P, i64, P]),
+        "tmvp_kem_encaps": (i, [P, i, i, P, P, P, P, i64, P]),
+        "tmvp_kem_decaps": (i, [P, i, i, P, P, P, P, i64, P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -301,6 +305,28 @@ class Context:
             raise TmvpError(rc, "tmvp_innerprod")
 
 
+    def kem_keypair(self, pk, sk, seeds, batch, param_set=SABER, backend=MFMA, stream=None):
+        ps = PARAM_SET_NAMES.get(param_set, param_set)
+        rc = load_library().tmvp_kem_keypair(self.handle, int(ps), backend, _ptr(pk), _ptr(sk), _ptr(seeds),
+                                             int(batch), _stream(stream))
+        if rc:
+            raise TmvpError(rc, "tmvp_kem_keypair")
+
+    def kem_encaps(self, ct, ss, pk, rnd, batch, param_set=SABER, backend=MFMA, stream=None):
+        ps = PARAM_SET_NAMES.get(param_set, param_set)
+        rc = load_library().tmvp_kem_encaps(self.handle, int(ps), backend, _ptr(ct), _ptr(ss), _ptr(pk), _ptr(rnd),
+                                            int(batch), _stream(stream))
+        if rc:
+            raise TmvpError(rc, "tmvp_kem_encaps")
+
+    def kem_decaps(self, ss, ct, sk, batch, param_set=SABER, backend=MFMA, fail=None, stream=None):
+        ps = PARAM_SET_NAMES.get(param_set, param_set)
+        rc = load_library().tmvp_kem_decaps(self.handle, int(ps), backend, _ptr(ss), _ptr(ct), _ptr(sk),
+                                            _ptr(fail), int(batch), _stream(stream))
+        if rc:
+            raise TmvpError(rc, "tmvp_kem_decaps")
+
+
 # ---- numpy conveniences (allocate, copy, run, copy back, synchronise) ----
 
 def run_polymul(ctx, a, s, eq_bits=13, backend=MFMA):
@@ -343,3 +369,41 @@ def run_innerprod(ctx, b, s, param_set=SABER, backend=MFMA, epilogue=IP_RAW, aux
     device_synchronize()
     ctx.check()
     return dout.download(np.uint16, (batch, N))
+
+
+def run_kem_keypair(ctx, seeds, param_set=SABER, backend=MFMA):
+    p = get_params(param_set)
+    seeds = np.ascontiguousarray(seeds, np.uint8).reshape(-1, 96)
+    batch = seeds.shape[0]
+    dseeds = DeviceBuffer.from_numpy(seeds)
+    dpk, dsk = DeviceBuffer(batch * p["pk_bytes"]), DeviceBuffer(batch * p["sk_bytes"])
+    ctx.kem_keypair(dpk, dsk, dseeds, batch, param_set, backend)
+    device_synchronize()
+    ctx.check()
+    return dpk.download(np.uint8, (batch, -1)), dsk.download(np.uint8, (batch, -1))
+
+
+def run_kem_encaps(ctx, pk, rnd, param_set=SABER, backend=MFMA):
+    p = get_params(param_set)
+    pk = np.ascontiguousarray(pk, np.uint8).reshape(-1, p["pk_bytes"])
+    rnd = np.ascontiguousarray(rnd, np.uint8).reshape(-1, 32)
+    batch = pk.shape[0]
+    dpk, drnd = DeviceBuffer.from_numpy(pk), DeviceBuffer.from_numpy(rnd)
+    dct, dss = DeviceBuffer(batch * p["ct_bytes"]), DeviceBuffer(batch * 32)
+    ctx.kem_encaps(dct, dss, dpk, drnd, batch, param_set, backend)
+    device_synchronize()
+    ctx.check()
+    return dct.download(np.uint8, (batch, -1)), dss.download(np.uint8, (batch, -1))
+
+
+def run_kem_decaps(ctx, ct, sk, param_set=SABER, backend=MFMA):
+    p = get_params(param_set)
+    ct = np.ascontiguousarray(ct, np.uint8).reshape(-1, p["ct_bytes"])
+    sk = np.ascontiguousarray(sk, np.uint8).reshape(-1, p["sk_bytes"])
+    batch = ct.shape[0]
+    dct, dsk = DeviceBuffer.from_numpy(ct), DeviceBuffer.from_numpy(sk)
+    dss, dfail = DeviceBuffer(batch * 32), DeviceBuffer(batch)
+    ctx.kem_decaps(dss, dct, dsk, batch, param_set, backend, fail=dfail)
+    device_synchronize()
+    ctx.check()
+    return dss.download(np.uint8, (batch, -1)), dfail.download(np.uint8)

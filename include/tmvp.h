@@ -135,6 +135,36 @@ int tmvp_innerprod(tmvp_ctx *ctx, int param_set, int backend, int epilogue, uint
                    const uint16_t *d_b, const uint16_t *d_s, const uint16_t *d_aux,
                    int64_t batch, int flags, tmvp_stream_t stream);
 
+/* ---- batched KEM (fused GPU pipeline: SHAKE-128 GenMatrix, CBD GenSecret,
+ * MatrixVectorMul, InnerProd, rounding, codecs, Fujisaki-Okamoto hashes) ----
+ * Byte layouts are the reference's: pk = pack_EP(b) | seed_A,
+ * sk = s | pk | H(pk) | z (Saber s as 13-bit words, Sable as 2-bit codes),
+ * ct = pack_EP(b') | pack_ET(v').  Buffers are [batch][size] device arrays.
+ * Scratch is owned by the context, one workspace per stream; it grows on
+ * demand (which synchronises the device once) and is reused afterwards. */
+
+/* Key generation from per-instance seeds [batch][96] = seed_A | seed_s | z.
+ * The reference has no key generation; this is Saber round 3
+ * (seed_A' = SHAKE-128(seed_A), b = round(A^T s)), the form the reference's
+ * test-vector keys (S/include/tv.h:8-10, SB/include/tv.h:15-19) satisfy. */
+int tmvp_kem_keypair(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_pk, uint8_t *d_sk,
+                     const uint8_t *d_seeds, int64_t batch, tmvp_stream_t stream);
+
+/* Encapsulation with per-instance randomness d_rnd [batch][32].
+ * Replaces saber_enc's device chain (S/src/cuda_kernel.cu:114-186) and
+ * SABLE_enc's (SB/src/SABLE_indcpa.cu:133-233): m = H(rnd),
+ * kr = G(m | H(pk)), ct = Enc(pk, m; kr[32:]), ss = H(kr[0:32] | H(ct)). */
+int tmvp_kem_encaps(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_ct, uint8_t *d_ss,
+                    const uint8_t *d_pk, const uint8_t *d_rnd, int64_t batch, tmvp_stream_t stream);
+
+/* Decapsulation.  Replaces saber_dec's device chain
+ * (S/src/cuda_kernel.cu:341-450) and SABLE_dec's (SB/src/SABLE_indcpa.cu:422-563),
+ * with a working implicit rejection: if the re-encryption differs from ct,
+ * ss = H(z | H(ct)).  d_fail (optional, [batch] bytes) receives 1 for a
+ * rejected ciphertext. */
+int tmvp_kem_decaps(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_ss, const uint8_t *d_ct,
+                    const uint8_t *d_sk, uint8_t *d_fail, int64_t batch, tmvp_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
