@@ -85,6 +85,7 @@ KemParams kem_params(int ps)
     P.pk = P.pvc + 32;
     P.sk = P.indcpa_sk + P.pk + 64;
     P.ct = P.pvc + p.et * 32;
+    P.araw = p.l * p.l * p.eq * 32;
     return P;
 }
 

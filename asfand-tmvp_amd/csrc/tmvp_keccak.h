@@ -104,29 +104,32 @@ __device__ __forceinline__ void f1600(uint64_t (&A)[25])
     }
 }
 
-// Absorb `len` bytes (8-byte aligned source) with rate `rate` and domain
-// byte `ds`, leaving the state ready for the first squeeze permutation.
-__device__ __forceinline__ void absorb(uint64_t (&A)[25], const uint8_t *in, int len, int rate, uint8_t ds)
+// Absorb `len` bytes (8-byte aligned source) with rate RATE and domain byte
+// `ds`, leaving the state ready for the first squeeze permutation.  Every
+// state index is a compile-time constant so the state stays in registers.
+template <int RATE> __device__ __forceinline__ void absorb(uint64_t (&A)[25], const uint8_t *in, int len, uint8_t ds)
 {
+    constexpr int RW = RATE / 8;
 #pragma unroll
     for (int i = 0; i < 25; i++) A[i] = 0;
     const uint64_t *w = reinterpret_cast<const uint64_t *>(in);
-    const int rw = rate / 8;
-    while (len >= rate) {
-        for (int i = 0; i < rw; i++) A[i] ^= w[i];
+    while (len >= RATE) {
+#pragma unroll
+        for (int i = 0; i < RW; i++) A[i] ^= w[i];
         f1600(A);
-        w += rw;
-        len -= rate;
+        w += RW;
+        len -= RATE;
     }
-    const int full = len / 8;
-    for (int i = 0; i < full; i++) A[i] ^= w[i];
-    uint64_t last = 0;
-    const int rem = len & 7;
+    const int full = len >> 3, rem = len & 7;
+    uint64_t last = (uint64_t)ds << (8 * rem);
     const uint8_t *tail = reinterpret_cast<const uint8_t *>(w + full);
     for (int b = 0; b < rem; b++) last |= (uint64_t)tail[b] << (8 * b);
-    last |= (uint64_t)ds << (8 * rem);
-    A[full] ^= last;
-    A[rw - 1] ^= 0x8000000000000000ull;
+#pragma unroll
+    for (int i = 0; i < RW; i++) {
+        if (i < full) A[i] ^= w[i];
+        else if (i == full) A[i] ^= last;
+    }
+    A[RW - 1] ^= 0x8000000000000000ull;
 }
 
 // Absorb a 32-byte value held in four lanes.
@@ -161,7 +164,7 @@ constexpr uint8_t kSha3DS = 0x06;
 __device__ __forceinline__ void sha3_256(uint64_t (&out)[4], const uint8_t *in, int len)
 {
     uint64_t A[25];
-    absorb(A, in, len, kSha3_256Rate, kSha3DS);
+    absorb<kSha3_256Rate>(A, in, len, kSha3DS);
     f1600(A);
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = A[i];
@@ -192,6 +195,42 @@ __device__ __forceinline__ void sha3_512_64(uint64_t (&out)[8], const uint64_t (
     f1600(A);
 #pragma unroll
     for (int i = 0; i < 8; i++) out[i] = A[i];
+}
+
+// Squeeze the sponge (absorbed, not yet permuted) as a stream of `count`
+// w-bit values, w <= 16, least significant bit first, calling sink(k, v) for
+// value k.  Blocks are consumed as 42 half-lanes in a fully unrolled loop, so
+// no state word is indexed at run time.
+template <class Sink> __device__ __forceinline__ void squeeze_values(uint64_t (&A)[25], int w, int count, Sink &&sink)
+{
+    const uint32_t mask = (1u << w) - 1u;
+    uint64_t buf = 0;
+    int nbits = 0, k = 0;
+    while (k < count) {
+        f1600(A);
+#pragma unroll
+        for (int h = 0; h < kShake128Rate / 4; h++) {
+            buf |= (uint64_t)(uint32_t)(A[h >> 1] >> (32 * (h & 1))) << nbits;
+            nbits += 32;
+            while (nbits >= w && k < count) {
+                sink(k, (uint32_t)buf & mask);
+                buf >>= w;
+                nbits -= w;
+                k++;
+            }
+        }
+    }
+}
+
+// Squeeze `nwords` 64-bit words of output (absorbed, not yet permuted state).
+__device__ __forceinline__ void squeeze_words(uint64_t (&A)[25], uint64_t *dst, int nwords)
+{
+    for (int k = 0; k < nwords; k += kShake128Rate / 8) {
+        f1600(A);
+#pragma unroll
+        for (int i = 0; i < kShake128Rate / 8; i++)
+            if (k + i < nwords) dst[k + i] = A[i];
+    }
 }
 
 } // namespace keccak

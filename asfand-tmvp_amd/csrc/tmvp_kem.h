@@ -14,12 +14,13 @@ struct KemParams {
     int pvc;       // L * EP * 32: compressed polynomial vector
     int indcpa_sk; // Saber: L * 13 * 32, Sable: L * 2 * 32
     int pk, sk, ct;
+    int araw; // L * L * EQ * 32: the SHAKE-128 stream GenMatrix unpacks
 };
 
 // Scratch for one batched call (carved from one device allocation).
 struct KemWorkspace {
     uint16_t *A, *s, *v1, *v2, *bits, *poly;
-    uint8_t *buf, *kr, *ctp;
+    uint8_t *buf, *kr, *ctp, *araw;
 };
 
 size_t kem_workspace_bytes(const KemParams &P, int64_t B);

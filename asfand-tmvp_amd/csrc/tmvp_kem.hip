@@ -29,45 +29,6 @@ using namespace keccak;
 
 // ---------------------------------------------------------------- sampling
 
-// Bit-stream reader over the output of a SHAKE-128 squeeze.
-struct ShakeReader {
-    uint64_t A[25];
-    int lane;     // next lane of the current block
-    uint64_t buf; // pending bits, LSB first
-    int nbits;
-
-    __device__ __forceinline__ void start(const uint64_t (&seed)[4])
-    {
-        absorb32(A, seed, kShake128Rate, kShakeDS);
-        f1600(A);
-        lane = 0;
-        buf = 0;
-        nbits = 0;
-    }
-    // Make at least `need` (<= 32) bits available.
-    __device__ __forceinline__ void refill(int need)
-    {
-        while (nbits < need) {
-            if (lane == kShake128Rate / 8 * 2) { // 21 lanes = 42 half-lanes per block
-                f1600(A);
-                lane = 0;
-            }
-            const uint32_t half = (uint32_t)(A[lane >> 1] >> (32 * (lane & 1)));
-            buf |= (uint64_t)half << nbits;
-            nbits += 32;
-            lane++;
-        }
-    }
-    __device__ __forceinline__ uint32_t take(int w)
-    {
-        refill(w);
-        const uint32_t v = (uint32_t)(buf & ((1ull << w) - 1));
-        buf >>= w;
-        nbits -= w;
-        return v;
-    }
-};
-
 __device__ __forceinline__ void load32(uint64_t (&m)[4], const uint8_t *p)
 {
     const uint64_t *w = reinterpret_cast<const uint64_t *>(p);
@@ -81,111 +42,224 @@ __device__ __forceinline__ void store32(uint8_t *p, const uint64_t (&m)[4])
     for (int i = 0; i < 4; i++) w[i] = m[i];
 }
 
-// Threads [0, B): GenMatrix of instance t from the 32-byte seed at
-// seed_a + t * a_stride  -> A[t] = L*L EQ-bit polynomials, row-major
-// (S/src/pack.cu:145-153, SB/src/pack.cu:41-50).
-// Threads [B, 2B): GenSecret of instance t - B from seed_s + t * s_stride
-// -> S[t] = L CBD_mu polynomials (S/src/poly.cu:836-916, :1062-1073;
-// SB/src/fips202.cu:625-668), Saber as int16, Sable masked mod q.
-__global__ __launch_bounds__(64) void gen_kernel(KemParams P, const uint8_t *seed_a, int a_stride,
-                                                 const uint8_t *seed_s, int s_stride, uint16_t *A, uint16_t *S,
-                                                 int64_t B)
+// Collects 16-bit values four at a time into 8-byte stores.
+struct Quad {
+    uint2 *dst;
+    uint64_t acc = 0;
+    __device__ __forceinline__ void put(int k, uint32_t v)
+    {
+        acc |= (uint64_t)v << (16 * (k & 3));
+        if ((k & 3) == 3) {
+            dst[k >> 2] = make_uint2((uint32_t)acc, (uint32_t)(acc >> 32));
+            acc = 0;
+        }
+    }
+};
+
+// GenMatrix, first half: the SHAKE-128(seed) stream of L*L*EQ*32 bytes, which
+// the EQ-bit unpacker turns into L*L polynomials, row-major
+// (S/src/pack.cu:145-153, SB/src/pack.cu:41-50).  Keeping the sponge to
+// permute-and-store leaves the unpacking to a kernel with one thread per 64
+// coefficients.
+__device__ __forceinline__ void gen_matrix_stream(const KemParams &P, const uint64_t (&seed)[4], uint8_t *dst)
+{
+    uint64_t A[25];
+    absorb32(A, seed, kShake128Rate, kShakeDS);
+    squeeze_words(A, reinterpret_cast<uint64_t *>(dst), P.araw / 8);
+}
+
+// GenSecret: SHAKE-128(seed) read as L*256 mu-bit groups, each the popcount of
+// its low half minus that of its high half (S/src/poly.cu:836-916, :1062-1073;
+// SB/src/fips202.cu:625-668); Saber stores int16, Sable the value mod q.
+__device__ __forceinline__ void gen_secret(const KemParams &P, const uint64_t (&seed)[4], uint16_t *dst)
+{
+    uint64_t A[25];
+    absorb32(A, seed, kShake128Rate, kShakeDS);
+    const int half = P.mu / 2;
+    const uint32_t lmask = (1u << half) - 1u, qmask = P.sable ? (1u << P.eq) - 1u : 0xffffu;
+    Quad q{reinterpret_cast<uint2 *>(dst)};
+    squeeze_values(A, P.mu, P.l * kN, [&](int k, uint32_t bits) {
+        const int v = __builtin_popcount(bits & lmask) - __builtin_popcount(bits >> half);
+        q.put(k, (uint32_t)v & qmask);
+    });
+}
+
+// H(pk) | m -> G -> kr; the shared first step of encapsulation and of the
+// re-encryption in decapsulation (S/src/cuda_kernel.cu:116-118, :378-379).
+__device__ __forceinline__ void hash_g(const uint64_t (&m)[4], const uint64_t (&hpk)[4], uint8_t *buf, uint8_t *kr,
+                                       uint64_t (&coins)[4])
+{
+    uint64_t b[8], k[8];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        b[i] = m[i];
+        b[4 + i] = hpk[i];
+    }
+    sha3_512_64(k, b);
+    uint64_t *bo = reinterpret_cast<uint64_t *>(buf);
+    uint64_t *ko = reinterpret_cast<uint64_t *>(kr);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        bo[i] = b[i];
+        ko[i] = k[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) coins[i] = k[4 + i];
+}
+
+// The front kernels run the long matrix sponge (threads [0, B)) beside the
+// rest of the instance's hashing and secret sampling (threads [B, 2B)); the
+// matrix stream is unpacked afterwards.
+
+// Key generation: seeds[t] = seed_A | seed_s | z.  seed_A' = SHAKE-128(seed_A)
+// goes to the pk tail, A = GenMatrix(seed_A'); s = GenSecret(seed_s); z goes
+// to the sk tail.
+__global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
+                                                      uint8_t *araw, uint16_t *S, int64_t B)
 {
     const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (t >= 2 * B) return;
-    ShakeReader rd;
-    uint64_t seed[4];
+    uint64_t m[4];
     if (t < B) {
-        load32(seed, seed_a + t * a_stride);
-        rd.start(seed);
-        const int n = P.l * P.l * kN;
-        uint2 *dst = reinterpret_cast<uint2 *>(A + t * n);
-        for (int k = 0; k < n; k += 4) {
-            const uint32_t c0 = rd.take(P.eq), c1 = rd.take(P.eq), c2 = rd.take(P.eq), c3 = rd.take(P.eq);
-            dst[k >> 2] = make_uint2(c0 | (c1 << 16), c2 | (c3 << 16));
-        }
+        uint64_t st[25];
+        load32(m, seeds + t * 96);
+        absorb32(st, m, kShake128Rate, kShakeDS);
+        f1600(st);
+        const uint64_t sa[4] = {st[0], st[1], st[2], st[3]};
+        store32(pk + t * P.pk + P.pvc, sa);
+        gen_matrix_stream(P, sa, araw + t * P.araw);
     } else {
         const int64_t i = t - B;
-        load32(seed, seed_s + i * s_stride);
-        rd.start(seed);
-        const int n = P.l * kN, half = P.mu / 2;
-        const uint32_t qmask = (1u << P.eq) - 1u;
-        uint2 *dst = reinterpret_cast<uint2 *>(S + i * n);
-        for (int k = 0; k < n; k += 4) {
-            uint32_t c[4];
+        load32(m, seeds + i * 96 + 32);
+        gen_secret(P, m, S + i * (P.l * kN));
+        load32(m, seeds + i * 96 + 64);
+        store32(sk + i * P.sk + P.sk - 32, m);
+    }
+}
+
+// Encapsulation: A from the pk tail; m = H(rnd), kr = G(m | H(pk)), s' =
+// GenSecret(kr[32:64]).
+__global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
+                                                       uint8_t *buf, uint8_t *kr, uint8_t *araw, uint16_t *S,
+                                                       int64_t B)
+{
+    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (t >= 2 * B) return;
+    uint64_t m[4];
+    if (t < B) {
+        load32(m, pk + t * P.pk + P.pvc);
+        gen_matrix_stream(P, m, araw + t * P.araw);
+    } else {
+        const int64_t i = t - B;
+        uint64_t r[4], h[4], coins[4];
+        load32(r, rnd + i * 32);
+        sha3_256_32(m, r);
+        sha3_256(h, pk + i * P.pk, P.pk);
+        hash_g(m, h, buf + i * 64, kr + i * 64, coins);
+        gen_secret(P, coins, S + i * (P.l * kN));
+    }
+}
+
+// Decapsulation, after the decryption: A from the pk inside the sk; m' from
+// its bits, kr = G(m' | H(pk)) with H(pk) from the sk, s' = GenSecret(kr[32:64]).
+__global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
+                                                       uint8_t *buf, uint8_t *kr, uint8_t *araw, uint16_t *S,
+                                                       int64_t B)
+{
+    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (t >= 2 * B) return;
+    uint64_t m[4];
+    if (t < B) {
+        load32(m, sk + t * P.sk + P.indcpa_sk + P.pvc);
+        gen_matrix_stream(P, m, araw + t * P.araw);
+    } else {
+        const int64_t i = t - B;
+        const uint32_t *mb = reinterpret_cast<const uint32_t *>(mbits + i * kN); // bit k in uint16 k
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t bits = rd.take(P.mu);
-                const int v = __builtin_popcount(bits & ((1u << half) - 1u)) - __builtin_popcount(bits >> half);
-                c[u] = P.sable ? ((uint32_t)v & qmask) : ((uint32_t)v & 0xffffu);
+        for (int w = 0; w < 4; w++) {
+            uint64_t x = 0;
+#pragma unroll
+            for (int j = 0; j < 32; j++) {
+                const uint32_t two = mb[32 * w + j];
+                x |= (uint64_t)(two & 1u) << (2 * j) | (uint64_t)((two >> 16) & 1u) << (2 * j + 1);
             }
-            dst[k >> 2] = make_uint2(c[0] | (c[1] << 16), c[2] | (c[3] << 16));
+            m[w] = x;
         }
+        uint64_t h[4], coins[4];
+        load32(h, sk + i * P.sk + P.sk - 64);
+        hash_g(m, h, buf + i * 64, kr + i * 64, coins);
+        gen_secret(P, coins, S + i * (P.l * kN));
     }
 }
 
 // ---------------------------------------------------------------- codecs
 
-// One thread per (instance, group of 8 coefficients):
-//   pack:   8 w-bit values -> w bytes;  unpack: w bytes -> 8 values.
+// One thread per (instance, chunk of 64 coefficients), which is exactly W
+// 64-bit words of a W-bit little-endian bit stream (LSB first, as every
+// reference codec: S/src/pack.cu:1-213, SB/src/pack.cu:182-467).  W is a
+// template argument so every shift and word index is a constant.
 // kXfSk2 is the Sable secret-key code: s & 3 on packing, ((x ^ 2) - 2) on
 // unpacking (SB/src/pack.cu:366-377).
 enum Xf : int { kXfNone = 0, kXfSk2 = 1 };
 
+template <int W, int XF>
 __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t src_stride, uint8_t *dst,
-                                                   int64_t dst_stride, int dst_off, int w, int groups, int xf,
-                                                   int64_t B)
+                                                   int64_t dst_stride, int dst_off, int chunks, int64_t B)
 {
     const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= B * groups) return;
-    const int64_t t = gid / groups;
-    const int g = (int)(gid - t * groups);
-    const uint16_t *s = src + t * src_stride + g * 8;
-    const uint32_t mask = (1u << w) - 1u;
-    uint64_t lo = 0, hi = 0;
+    if (gid >= B * chunks) return;
+    const int64_t t = gid / chunks;
+    const int c = (int)(gid - t * chunks);
+    const uint4 *s = reinterpret_cast<const uint4 *>(src + t * src_stride + c * 64);
+    constexpr uint32_t mask = (1u << W) - 1u;
+    uint64_t words[W];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        uint32_t v = s[i];
-        if (xf == kXfSk2) v &= 3u;
-        v &= mask;
-        const int bit = i * w;
-        if (bit < 64) {
-            lo |= (uint64_t)v << bit;
-            if (bit + w > 64) hi |= (uint64_t)v >> (64 - bit);
-        } else {
-            hi |= (uint64_t)v << (bit - 64);
+    for (int k = 0; k < W; k++) words[k] = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const uint4 v = s[q];
+        const uint32_t e[8] = {v.x & 0xffffu, v.x >> 16, v.y & 0xffffu, v.y >> 16,
+                               v.z & 0xffffu, v.z >> 16, v.w & 0xffffu, v.w >> 16};
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int bit = (8 * q + j) * W, wi = bit >> 6, o = bit & 63;
+            const uint64_t x = (XF == kXfSk2 ? (e[j] & 3u) : e[j]) & mask;
+            words[wi] |= x << o;
+            if (o + W > 64) words[wi + 1] |= x >> (64 - o);
         }
     }
-    uint8_t *d = dst + t * dst_stride + dst_off + (int64_t)g * w;
-    for (int b = 0; b < w; b++) d[b] = (uint8_t)(b < 8 ? (lo >> (8 * b)) : (hi >> (8 * (b - 8))));
+    uint64_t *d = reinterpret_cast<uint64_t *>(dst + t * dst_stride + dst_off + (int64_t)c * W * 8);
+#pragma unroll
+    for (int k = 0; k < W; k++) d[k] = words[k];
 }
 
+template <int W, int XF>
 __global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *src, int64_t src_stride, int src_off,
-                                                     uint16_t *dst, int64_t dst_stride, int w, int groups, int xf,
-                                                     int64_t B)
+                                                     uint16_t *dst, int64_t dst_stride, int chunks, int64_t B)
 {
     const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= B * groups) return;
-    const int64_t t = gid / groups;
-    const int g = (int)(gid - t * groups);
-    const uint8_t *s = src + t * src_stride + src_off + (int64_t)g * w;
-    uint64_t lo = 0, hi = 0;
-    for (int b = 0; b < w; b++) {
-        if (b < 8) lo |= (uint64_t)s[b] << (8 * b);
-        else hi |= (uint64_t)s[b] << (8 * (b - 8));
-    }
-    const uint32_t mask = (1u << w) - 1u;
-    uint16_t *d = dst + t * dst_stride + g * 8;
+    if (gid >= B * chunks) return;
+    const int64_t t = gid / chunks;
+    const int c = (int)(gid - t * chunks);
+    const uint64_t *s = reinterpret_cast<const uint64_t *>(src + t * src_stride + src_off + (int64_t)c * W * 8);
+    constexpr uint32_t mask = (1u << W) - 1u;
+    uint64_t words[W];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const int bit = i * w;
-        uint64_t v;
-        if (bit + w <= 64) v = lo >> bit;
-        else if (bit >= 64) v = hi >> (bit - 64);
-        else v = (lo >> bit) | (hi << (64 - bit));
-        uint32_t x = (uint32_t)v & mask;
-        if (xf == kXfSk2) x = ((x ^ 2u) - 2u) & 0xffffu;
-        d[i] = (uint16_t)x;
+    for (int k = 0; k < W; k++) words[k] = s[k];
+    uint4 *d = reinterpret_cast<uint4 *>(dst + t * dst_stride + c * 64);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        uint32_t e[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int bit = (8 * q + j) * W, wi = bit >> 6, o = bit & 63;
+            uint64_t x = words[wi] >> o;
+            if (o + W > 64) x |= words[wi + 1] << (64 - o);
+            uint32_t v = (uint32_t)x & mask;
+            if (XF == kXfSk2) v = ((v ^ 2u) - 2u) & 0xffffu;
+            e[j] = v;
+        }
+        d[q] = make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16), e[6] | (e[7] << 16));
     }
 }
 
@@ -204,23 +278,6 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint8_t *src, int64_t s
 
 // ---------------------------------------------------------------- hashing steps (one thread per instance)
 
-// Key generation, before sampling: seed_A' = SHAKE-128(seed_A) into the pk
-// tail, z into the sk tail.  seeds[t] = seed_A | seed_s | z.
-__global__ __launch_bounds__(64) void kp_seed_kernel(KemParams P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
-                                                     int64_t B)
-{
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (t >= B) return;
-    uint64_t m[4], A[25];
-    load32(m, seeds + t * 96);
-    absorb32(A, m, kShake128Rate, kShakeDS);
-    f1600(A);
-    uint64_t o[4] = {A[0], A[1], A[2], A[3]};
-    store32(pk + t * P.pk + P.pvc, o);
-    load32(m, seeds + t * 96 + 64);
-    store32(sk + t * P.sk + P.sk - 32, m);
-}
-
 // Key generation, last step: H(pk) into the sk.
 __global__ __launch_bounds__(64) void kp_hash_kernel(KemParams P, const uint8_t *pk, uint8_t *sk, int64_t B)
 {
@@ -229,51 +286,6 @@ __global__ __launch_bounds__(64) void kp_hash_kernel(KemParams P, const uint8_t 
     uint64_t h[4];
     sha3_256(h, pk + t * P.pk, P.pk);
     store32(sk + t * P.sk + P.sk - 64, h);
-}
-
-// Encapsulation, first step (S/src/cuda_kernel.cu:116-118):
-// m = H(rnd), buf = m | H(pk), kr = G(buf).
-__global__ __launch_bounds__(64) void enc_hash1_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
-                                                       uint8_t *buf, uint8_t *kr, int64_t B)
-{
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (t >= B) return;
-    uint64_t r[4], m[4], h[4], b[8], k[8];
-    load32(r, rnd + t * 32);
-    sha3_256_32(m, r);
-    sha3_256(h, pk + t * P.pk, P.pk);
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        b[i] = m[i];
-        b[4 + i] = h[i];
-    }
-    sha3_512_64(k, b);
-    uint64_t *bo = reinterpret_cast<uint64_t *>(buf + t * 64);
-    uint64_t *ko = reinterpret_cast<uint64_t *>(kr + t * 64);
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        bo[i] = b[i];
-        ko[i] = k[i];
-    }
-}
-
-// Message bits of buf[t][0:32] (BS2POLmsg_gpu, S/src/pack.cu:86-99): one
-// thread per (instance, byte).
-__global__ __launch_bounds__(256) void msg_bits_kernel(const uint8_t *buf, uint16_t *bits, int64_t B)
-{
-    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= B * 32) return;
-    const int64_t t = gid / 32;
-    const int byte = (int)(gid - t * 32);
-    const uint32_t v = buf[t * 64 + byte];
-    uint2 lo, hi;
-    lo.x = (v & 1u) | (((v >> 1) & 1u) << 16);
-    lo.y = ((v >> 2) & 1u) | (((v >> 3) & 1u) << 16);
-    hi.x = ((v >> 4) & 1u) | (((v >> 5) & 1u) << 16);
-    hi.y = ((v >> 6) & 1u) | (((v >> 7) & 1u) << 16);
-    uint2 *d = reinterpret_cast<uint2 *>(bits + t * kN + byte * 8);
-    d[0] = lo;
-    d[1] = hi;
 }
 
 // Encapsulation, last step (S/src/cuda_kernel.cu:185-186): kr[32:] = H(ct), ss = H(kr).
@@ -292,34 +304,6 @@ __global__ __launch_bounds__(64) void enc_hash2_kernel(KemParams P, const uint8_
     }
     sha3_256_64(s, k);
     store32(ss + t * 32, s);
-}
-
-// Decapsulation, after the decryption (S/src/cuda_kernel.cu:369-379):
-// m' from its bits, buf = m' | H(pk) (from the sk), kr = G(buf).
-__global__ __launch_bounds__(64) void dec_hash1_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
-                                                       uint8_t *buf, uint8_t *kr, int64_t B)
-{
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (t >= B) return;
-    uint64_t b[8], k[8];
-    const uint16_t *mb = mbits + t * kN;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        uint64_t w = 0;
-        for (int bit = 0; bit < 64; bit++) w |= (uint64_t)(mb[64 * i + bit] & 1u) << bit;
-        b[i] = w;
-    }
-    const uint64_t *h = reinterpret_cast<const uint64_t *>(sk + t * P.sk + P.sk - 64);
-#pragma unroll
-    for (int i = 0; i < 4; i++) b[4 + i] = h[i];
-    sha3_512_64(k, b);
-    uint64_t *bo = reinterpret_cast<uint64_t *>(buf + t * 64);
-    uint64_t *ko = reinterpret_cast<uint64_t *>(kr + t * 64);
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        bo[i] = b[i];
-        ko[i] = k[i];
-    }
 }
 
 // Decapsulation, last step: compare ct with the re-encryption, kr[32:] = H(ct),
@@ -381,38 +365,69 @@ static hipError_t product(const KemParams &P, int backend, int mode, int epi, ui
         if (e_ != hipSuccess) return e_;                                                                             \
     } while (0)
 
+// Codec launch for a run-time width: finds the instantiation with W == w.
+template <int W>
+static hipError_t pack_w(const uint16_t *src, int64_t src_stride, uint8_t *dst, int64_t dst_stride, int dst_off, int w,
+                         int chunks, int xf, int64_t B, hipStream_t st)
+{
+    if constexpr (W > 13) {
+        return hipErrorInvalidValue;
+    } else {
+        if (w != W) return pack_w<W + 1>(src, src_stride, dst, dst_stride, dst_off, w, chunks, xf, B, st);
+        const dim3 grid(blocks_for(B * chunks, 256));
+        if (xf == kXfSk2)
+            hipLaunchKernelGGL((pack_kernel<W, kXfSk2>), grid, dim3(256), 0, st, src, src_stride, dst, dst_stride,
+                               dst_off, chunks, B);
+        else
+            hipLaunchKernelGGL((pack_kernel<W, kXfNone>), grid, dim3(256), 0, st, src, src_stride, dst, dst_stride,
+                               dst_off, chunks, B);
+        return hipGetLastError();
+    }
+}
+
+template <int W>
+static hipError_t unpack_w(const uint8_t *src, int64_t src_stride, int src_off, uint16_t *dst, int64_t dst_stride,
+                           int w, int chunks, int xf, int64_t B, hipStream_t st)
+{
+    if constexpr (W > 13) {
+        return hipErrorInvalidValue;
+    } else {
+        if (w != W) return unpack_w<W + 1>(src, src_stride, src_off, dst, dst_stride, w, chunks, xf, B, st);
+        const dim3 grid(blocks_for(B * chunks, 256));
+        if (xf == kXfSk2)
+            hipLaunchKernelGGL((unpack_kernel<W, kXfSk2>), grid, dim3(256), 0, st, src, src_stride, src_off, dst,
+                               dst_stride, chunks, B);
+        else
+            hipLaunchKernelGGL((unpack_kernel<W, kXfNone>), grid, dim3(256), 0, st, src, src_stride, src_off, dst,
+                               dst_stride, chunks, B);
+        return hipGetLastError();
+    }
+}
+
+// ncoef is a multiple of 64; byte offsets and strides are multiples of 8.
 static hipError_t pack(const uint16_t *src, int64_t src_stride, uint8_t *dst, int64_t dst_stride, int dst_off, int w,
                        int ncoef, int xf, int64_t B, hipStream_t st)
 {
-    const int groups = ncoef / 8;
-    hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(B * groups, 256)), dim3(256), 0, st, src, src_stride, dst,
-                       dst_stride, dst_off, w, groups, xf, B);
-    return hipGetLastError();
+    return pack_w<1>(src, src_stride, dst, dst_stride, dst_off, w, ncoef / 64, xf, B, st);
 }
 
 static hipError_t unpack(const uint8_t *src, int64_t src_stride, int src_off, uint16_t *dst, int64_t dst_stride,
                          int w, int ncoef, int xf, int64_t B, hipStream_t st)
 {
-    const int groups = ncoef / 8;
-    hipLaunchKernelGGL(unpack_kernel, dim3(blocks_for(B * groups, 256)), dim3(256), 0, st, src, src_stride, src_off,
-                       dst, dst_stride, w, groups, xf, B);
-    return hipGetLastError();
+    return unpack_w<1>(src, src_stride, src_off, dst, dst_stride, w, ncoef / 64, xf, B, st);
 }
 
-// IND-CPA encryption of buf[t][0:32] with coins kr[t][32:64] under pk (at
-// pk + t * pk_stride) into ct (S/src/cuda_kernel.cu:124-178).
-static hipError_t indcpa_enc(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *pk,
-                             int64_t pk_stride, uint8_t *ct, int64_t B, int *status, hipStream_t st)
+// IND-CPA encryption, after sampling: A and s' are in the workspace, the
+// message in buf[t][0:32] (S/src/cuda_kernel.cu:124-178).  ct gets
+// pack_EP(round(A s')) | pack_ET(round(b . s' - m)).
+static hipError_t indcpa_enc_tail(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *pk,
+                                  int64_t pk_stride, uint8_t *ct, int64_t B, int *status, hipStream_t st)
 {
     const int L = P.l;
-    hipLaunchKernelGGL(gen_kernel, dim3(blocks_for(2 * B, 64)), dim3(64), 0, st, P, pk + P.pvc, (int)pk_stride,
-                       W.kr + 32, 64, W.A, W.s, B);
-    TRY(hipGetLastError());
     TRY(product(P, backend, kMatVec, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
     TRY(pack(W.v1, L * kN, ct, P.ct, 0, P.ep, L * kN, kXfNone, B, st));
     TRY(unpack(pk, pk_stride, 0, W.v2, L * kN, P.ep, L * kN, kXfNone, B, st));
-    hipLaunchKernelGGL(msg_bits_kernel, dim3(blocks_for(B * 32, 256)), dim3(256), 0, st, W.buf, W.bits, B);
-    TRY(hipGetLastError());
+    TRY(unpack(W.buf, 64, 0, W.bits, kN, 1, kN, kXfNone, B, st)); // message bits (S/src/pack.cu:86-99)
     TRY(product(P, backend, kInner, kEpiEnc, W.poly, W.v2, W.s, W.bits, B, status, st));
     TRY(pack(W.poly, kN, ct, P.ct, P.pvc, P.et, kN, kXfNone, B, st));
     return hipSuccess;
@@ -423,11 +438,10 @@ hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, c
 {
     if (B <= 0) return hipSuccess;
     const int L = P.l;
-    hipLaunchKernelGGL(kp_seed_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, seeds, pk, sk, B);
+    hipLaunchKernelGGL(kp_front_kernel, dim3(blocks_for(2 * B, 64)), dim3(64), 0, st, P, seeds, pk, sk, W.araw, W.s,
+                       B);
     TRY(hipGetLastError());
-    hipLaunchKernelGGL(gen_kernel, dim3(blocks_for(2 * B, 64)), dim3(64), 0, st, P, pk + P.pvc, P.pk, seeds + 32, 96,
-                       W.A, W.s, B);
-    TRY(hipGetLastError());
+    TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
     TRY(product(P, backend, kMatVecT, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
     TRY(pack(W.v1, L * kN, pk, P.pk, 0, P.ep, L * kN, kXfNone, B, st));
     if (P.sable)
@@ -445,9 +459,11 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
                       uint8_t *ct, uint8_t *ss, int64_t B, int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(enc_hash1_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, rnd, pk, W.buf, W.kr, B);
+    hipLaunchKernelGGL(enc_front_kernel, dim3(blocks_for(2 * B, 64)), dim3(64), 0, st, P, rnd, pk, W.buf, W.kr,
+                       W.araw, W.s, B);
     TRY(hipGetLastError());
-    TRY(indcpa_enc(P, W, backend, pk, P.pk, ct, B, status, st));
+    TRY(unpack(W.araw, P.araw, 0, W.A, P.l * P.l * kN, P.eq, P.l * P.l * kN, kXfNone, B, st));
+    TRY(indcpa_enc_tail(P, W, backend, pk, P.pk, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
 }
@@ -465,10 +481,12 @@ hipError_t kem_decaps(const KemParams &P, const KemWorkspace &W, int backend, co
     TRY(unpack(ct, P.ct, 0, W.v2, L * kN, P.ep, L * kN, kXfNone, B, st));
     TRY(unpack(ct, P.ct, P.pvc, W.bits, kN, P.et, kN, kXfNone, B, st));
     TRY(product(P, backend, kInner, kEpiDec, W.poly, W.v2, W.s, W.bits, B, status, st));
-    hipLaunchKernelGGL(dec_hash1_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, W.poly, sk, W.buf, W.kr, B);
+    // re-encryption of m' under the pk stored in the sk
+    hipLaunchKernelGGL(dec_front_kernel, dim3(blocks_for(2 * B, 64)), dim3(64), 0, st, P, W.poly, sk, W.buf, W.kr,
+                       W.araw, W.s, B);
     TRY(hipGetLastError());
-    // re-encryption under the pk stored in the sk
-    TRY(indcpa_enc(P, W, backend, sk + P.indcpa_sk, P.sk, W.ctp, B, status, st));
+    TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
+    TRY(indcpa_enc_tail(P, W, backend, sk + P.indcpa_sk, P.sk, W.ctp, B, status, st));
     hipLaunchKernelGGL(dec_fin_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, ct, W.ctp, sk, W.kr, ss, fail, B);
     return hipGetLastError();
 }
@@ -486,6 +504,7 @@ size_t kem_workspace_bytes(const KemParams &P, int64_t B)
     b += n * 64;             // buf
     b += n * 64;             // kr
     b += n * (size_t)P.ct;   // ctp
+    b += n * (size_t)P.araw; // araw
     return b + 16 * 256;
 }
 
@@ -506,6 +525,7 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
     W.buf = take(n * 64);
     W.kr = take(n * 64);
     W.ctp = take(n * (size_t)P.ct);
+    W.araw = take(n * (size_t)P.araw);
 }
 
 } // namespace tmvp

@@ -13,7 +13,12 @@ Timing uses HIP events recorded on the stream the kernels are launched on.
 `cpu_baseline` times the CPU oracle (a C restatement of the reference's
 Toom-4/schoolbook path) on a bounded sample of the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+`kem` reports the full batched Saber KEM at 16384 instances (key generation,
+encapsulation, decapsulation with all sampling, hashing, codecs and products on
+the GPU; seeds and keys resident in HBM): ops/s per step and p50/p99 of the
+GPU time of each batched call, after a round-trip check of every instance.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-kem] [--no-cpu]
 """
 import argparse
 import json
@@ -100,6 +105,91 @@ def cpu_baseline(seconds=12.0):
             "sample": f"{done} Saber L=3 MatrixVectorMul instances (Toom-4, {cores} threads, {dt:.1f} s)"}
 
 
+def percentile(xs, q):
+    xs = sorted(xs)
+    k = min(len(xs) - 1, max(0, int(np.ceil(q / 100.0 * len(xs))) - 1))
+    return xs[k]
+
+
+def kem_bench(ctx, stream, batch=16384, reps=40, ps="saber", cpu_seconds=4.0, with_cpu=True):
+    """Batched KEM: keypair, encaps and decaps of `batch` instances per call."""
+    p = T.get_params(ps)
+    rng = np.random.default_rng(3)
+    seeds = T.DeviceBuffer.from_numpy(rng.integers(0, 256, (batch, 96), dtype=np.uint8))
+    rnd = T.DeviceBuffer.from_numpy(rng.integers(0, 256, (batch, 32), dtype=np.uint8))
+    pk, sk = T.DeviceBuffer(batch * p["pk_bytes"]), T.DeviceBuffer(batch * p["sk_bytes"])
+    ct, ss, ss2 = T.DeviceBuffer(batch * p["ct_bytes"]), T.DeviceBuffer(batch * 32), T.DeviceBuffer(batch * 32)
+    fail = T.DeviceBuffer(batch)
+
+    def keypair():
+        ctx.kem_keypair(pk, sk, seeds, batch, ps, T.MFMA, stream=stream)
+
+    def encaps():
+        ctx.kem_encaps(ct, ss, pk, rnd, batch, ps, T.MFMA, stream=stream)
+
+    def decaps():
+        ctx.kem_decaps(ss2, ct, sk, batch, ps, T.MFMA, fail=fail, stream=stream)
+
+    keypair(); encaps(); decaps()
+    stream.synchronize()
+    ctx.check(stream)
+    ok = (not fail.download(np.uint8).any()) and np.array_equal(ss.download(np.uint8), ss2.download(np.uint8))
+    if not ok:
+        print("[bench] ERROR: KEM round trip failed", file=sys.stderr)
+        sys.exit(1)
+
+    ev = [T.Event() for _ in range(4)]
+    t_kp, t_enc, t_dec, t_pair = [], [], [], []
+    for _ in range(3):
+        keypair(); encaps(); decaps()
+    stream.synchronize()
+    for _ in range(reps):
+        ev[0].record(stream); keypair()
+        ev[1].record(stream); encaps()
+        ev[2].record(stream); decaps()
+        ev[3].record(stream)
+        ev[3].synchronize()
+        t_kp.append(ev[0].elapsed_ms(ev[1]))
+        t_enc.append(ev[1].elapsed_ms(ev[2]))
+        t_dec.append(ev[2].elapsed_ms(ev[3]))
+        t_pair.append(ev[1].elapsed_ms(ev[3]))
+    for e in ev:
+        e.destroy()
+    ctx.check(stream)
+    out = {
+        "workload": f"Saber KEM keypair/encaps/decaps, {batch} instances per call",
+        "batch": batch, "reps": reps,
+        "keypair_per_s": batch / (np.mean(t_kp) / 1e3),
+        "encaps_per_s": batch / (np.mean(t_enc) / 1e3),
+        "decaps_per_s": batch / (np.mean(t_dec) / 1e3),
+        "encaps_decaps_per_s": batch / (np.mean(t_pair) / 1e3),
+        "keypair_ms": {"p50": percentile(t_kp, 50), "p99": percentile(t_kp, 99)},
+        "encaps_ms": {"p50": percentile(t_enc, 50), "p99": percentile(t_enc, 99)},
+        "decaps_ms": {"p50": percentile(t_dec, 50), "p99": percentile(t_dec, 99)},
+        "encaps_decaps_ms": {"p50": percentile(t_pair, 50), "p99": percentile(t_pair, 99)},
+        "round_trip_ok": ok,
+    }
+    for b in (seeds, rnd, pk, sk, ct, ss, ss2, fail):
+        b.free()
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # test infrastructure: used only as the timed CPU baseline
+
+        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        n = 1024
+        seeds_h = np.random.default_rng(4).integers(0, 256, (n, 96), dtype=np.uint8)
+        rnd_h = np.random.default_rng(5).integers(0, 256, (n, 32), dtype=np.uint8)
+        pk_h, sk_h = O.kem_keypair_batch(ps, seeds_h, nthreads=cores)
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            ct_h, _ = O.kem_enc_batch(ps, rnd_h, pk_h, nthreads=cores)
+            O.kem_dec_batch(ps, ct_h, sk_h, nthreads=cores)
+            done += n
+        out["cpu_oracle_encaps_decaps_per_s"] = done / (time.perf_counter() - t0)
+        out["cpu_oracle_cores"] = cores
+    return out
+
+
 def calib_copy(mib):
     """A plain copy of known size: read mib MiB, write mib MiB (runtime copy kernel).
     Its FETCH_SIZE / WRITE_SIZE readings give the scale of those counters."""
@@ -119,6 +209,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-kem", action="store_true", help="skip the batched KEM leg")
+    ap.add_argument("--kem-batch", type=int, default=16384)
+    ap.add_argument("--kem-reps", type=int, default=40)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -165,7 +258,7 @@ def main():
     valu_value = args.batch / (ms_valu / valu_steps / 1e3)
     achieved = BYTES_PER_INSTANCE * args.batch / per_launch_s / 1e9
     result = {
-        "metric": "Saber MatVec TMVP ops/s at B=65536 (batched MFMA vs plain VALU)",
+        "metric": "Saber MatVec TMVP ops/s at B=65536 (batched MFMA vs plain VALU); KEM p99 ms",
         "value": value,
         "unit": "MatVec/s",
         "n_gpus": 1,
@@ -188,6 +281,11 @@ def main():
         "mfma_tops_useful": MACS_PER_INSTANCE * 2 * value / 1e12,
         "outputs_equal": same,
     }
+    if not args.no_kem:
+        print(f"[bench] KEM B={args.kem_batch} reps={args.kem_reps} ...", file=sys.stderr, flush=True)
+        result["kem"] = kem_bench(ctx, stream, args.kem_batch, args.kem_reps, with_cpu=not args.no_cpu)
+        print(f"[bench] KEM encaps+decaps p99 {result['kem']['encaps_decaps_ms']['p99']:.2f} ms", file=sys.stderr,
+              flush=True)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline()
