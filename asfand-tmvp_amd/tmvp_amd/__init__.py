@@ -64,7 +64,9 @@ _lib = None
 # The exported C-ABI symbols, as declared in include/tmvp.h.
 EXPORTS = ("tmvp_init", "tmvp_destroy", "tmvp_ctx_status", "tmvp_strerror", "tmvp_get_params",
            "tmvp_version", "tmvp_polymul", "tmvp_matvec", "tmvp_innerprod",
-           "tmvp_kem_keypair", "tmvp_kem_encaps", "tmvp_kem_decaps")
+           "tmvp_kem_keypair", "tmvp_kem_encaps", "tmvp_kem_decaps",
+           "tmvp_host_configure", "tmvp_host_set_randomness", "tmvp_host_status", "tmvp_host_release",
+           "saber_enc", "saber_dec", "SABLE_enc", "SABLE_dec")
 
 
 def load_library(path=LIB_PATH):
@@ -89,6 +91,14 @@ def load_library(path=LIB_PATH):
         "tmvp_kem_keypair": (i, [P, i, i, P, P, P, i64, P]),
         "tmvp_kem_encaps": (i, [P, i, i, P, P, P, P, i64, P]),
         "tmvp_kem_decaps": (i, [P, i, i, P, P, P, P, i64, P]),
+        "tmvp_host_configure": (i, [i, i64, i, i64, i]),
+        "tmvp_host_set_randomness": (i, [P, i64]),
+        "tmvp_host_status": (i, []),
+        "tmvp_host_release": (i, []),
+        "saber_enc": (None, [ctypes.c_uint8, P, P, P]),
+        "saber_dec": (None, [ctypes.c_uint8, P, P, P, P]),
+        "SABLE_enc": (None, [P, P, P, i]),
+        "SABLE_dec": (None, [P, P, P, P, i]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)

@@ -165,6 +165,30 @@ int tmvp_kem_encaps(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_ct, ui
 int tmvp_kem_decaps(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_ss, const uint8_t *d_ct,
                     const uint8_t *d_sk, uint8_t *d_fail, int64_t batch, tmvp_stream_t stream);
 
+/* ---- the reference's host entry points (same names and signatures) ----
+ * saber_enc / saber_dec (S/include/cuda_kernel.cuh:6-7) and SABLE_enc /
+ * SABLE_dec (SB/include/SABLE_indcpa.cuh:8-10) over the batched KEM above.
+ * The reference fixes the batch (BATCH: 1 for Saber, S/include/cuda_kernel.cuh:1;
+ * 512 for Sable, SB/include/SABLE_indcpa.cuh:6) and the parameter set at compile
+ * time and reads its keys and randomness from tv.h; here the batch and set
+ * come from tmvp_host_configure() (defaults: device 0, Saber L=3 x 1,
+ * Sable type 2 x 512), keys from the caller's buffers, and randomness from
+ * tmvp_host_set_randomness() (used by the next encapsulation only) or else
+ * from the operating system.  mode: 0 = MFMA, 1 = VALU, 2 = schoolbook VALU
+ * (S/main.cu:25-33).  Host buffers are [batch][size]: h_pk pk_bytes, h_sk
+ * sk_bytes, h_c ct_bytes, h_k 32.  h_pk is not read by the decapsulations (the
+ * sk holds the public key).  The functions return nothing, as in the
+ * reference; tmvp_host_status() gives the status of the last one.  Calls are
+ * serialised on one library-owned context and stream. */
+int tmvp_host_configure(int device, int64_t saber_batch, int saber_set, int64_t sable_batch, int sable_set);
+int tmvp_host_set_randomness(const uint8_t *h_rnd, int64_t batch);
+int tmvp_host_status(void);
+int tmvp_host_release(void);
+void saber_enc(uint8_t mode, uint8_t *h_k, uint8_t *h_pk, uint8_t *h_c);
+void saber_dec(uint8_t mode, uint8_t *h_c, uint8_t *h_pk, uint8_t *h_sk, uint8_t *h_k);
+void SABLE_enc(uint8_t *h_k, uint8_t *h_pk, uint8_t *h_c, int mode);
+void SABLE_dec(uint8_t *h_c, uint8_t *h_pk, uint8_t *h_sk, uint8_t *h_k, int mode);
+
 #ifdef __cplusplus
 }
 #endif

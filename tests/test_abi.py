@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "tmvp.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(tmvp_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?(\w+)\s*\(", text, flags=re.M)))
 
 
 def test_library_is_built_and_loads():
@@ -74,3 +74,21 @@ def test_init_without_gfx950_reports_nodev():
     h = ctypes.c_void_p()
     assert lib.tmvp_init(ctypes.byref(h), 0) == -6
     assert not h.value
+
+
+def test_reference_named_entry_points_without_gpu():
+    """saber_enc & co. keep the reference's void signatures; without a GPU they
+    record the failure in tmvp_host_status() instead of crashing."""
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    lib = T.load_library()
+    assert lib.tmvp_host_configure(0, 4, T.SABER, 8, T.SABLE2) == 0
+    assert lib.tmvp_host_configure(0, 0, T.SABER, 8, T.SABLE2) == -1
+    assert lib.tmvp_host_configure(0, 4, T.SABLE1, 8, T.SABLE2) == -1
+    assert lib.tmvp_host_configure(0, 4, T.SABER, 8, T.FIRESABER) == -1
+    pk, c, k = (ctypes.create_string_buffer(n) for n in (4 * 992, 4 * 1088, 4 * 32))
+    lib.saber_enc(0, k, pk, c)
+    assert lib.tmvp_host_status() == -6
+    lib.saber_enc(9, k, pk, c)
+    assert lib.tmvp_host_status() == -1
+    assert lib.tmvp_host_release() == 0
