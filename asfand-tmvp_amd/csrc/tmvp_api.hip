@@ -16,14 +16,31 @@ struct Workspace {
     hipStream_t stream;
     void *ptr;
     size_t bytes;
+    uint64_t used; // last-use stamp
+};
+// A measured plan for the multi-job MFMA launch of one work-list shape.
+constexpr int kMaxPlans = 32;
+struct Plan {
+    uint64_t sig; // 0 = empty
+    int wgs_per_cu;
+    float weight[kNumKinds];
 };
 } // namespace
 
 struct tmvp_ctx {
     int device;
+    int cus;
     int *d_status; // device word set by MFMA launches on out-of-range secrets
     Workspace ws[kMaxWorkspaces];
     int nws;
+    uint64_t ws_clock;
+    Plan plans[kMaxPlans];
+    int next_plan;
+};
+
+struct tmvp_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
 };
 
 namespace {
@@ -89,20 +106,37 @@ KemParams kem_params(int ps)
     return P;
 }
 
-// Workspace of at least `bytes` for `stream`; growing it synchronises the device.
+bool capturing(hipStream_t stream)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return stream && hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// Workspace of at least `bytes` for `stream`; growing it synchronises the
+// device.  A stream seen for the first time when all slots are taken gets the
+// least recently used slot, after a device synchronisation.
 int workspace(tmvp_ctx *ctx, hipStream_t stream, size_t bytes, uint8_t **out)
 {
     Workspace *w = nullptr;
     for (int i = 0; i < ctx->nws; i++)
         if (ctx->ws[i].stream == stream) w = &ctx->ws[i];
     if (!w) {
-        if (ctx->nws == kMaxWorkspaces) return TMVP_ENOMEM;
-        w = &ctx->ws[ctx->nws++];
+        if (capturing(stream)) return TMVP_EUNSUPPORTED; // reserve before capturing
+        if (ctx->nws < kMaxWorkspaces) {
+            w = &ctx->ws[ctx->nws++];
+            w->ptr = nullptr;
+            w->bytes = 0;
+        } else {
+            w = &ctx->ws[0];
+            for (int i = 1; i < kMaxWorkspaces; i++)
+                if (ctx->ws[i].used < w->used) w = &ctx->ws[i];
+            if (hipDeviceSynchronize() != hipSuccess) return TMVP_EHIP;
+        }
         w->stream = stream;
-        w->ptr = nullptr;
-        w->bytes = 0;
     }
+    w->used = ++ctx->ws_clock;
     if (w->bytes < bytes) {
+        if (capturing(stream)) return TMVP_EUNSUPPORTED; // no allocation inside a graph capture
         if (hipDeviceSynchronize() != hipSuccess) return TMVP_EHIP;
         if (w->ptr) (void)hipFree(w->ptr);
         w->ptr = nullptr;
@@ -144,6 +178,156 @@ Job base_job(tmvp_ctx *ctx, uint16_t *out, const uint16_t *a, const uint16_t *s,
     J.batch = batch;
     J.epi = kEpiMod;
     return J;
+}
+
+// Checks one work descriptor and builds its job, exactly as the matching
+// single-job entry point does.  Pointers are only required for batch > 0.
+int job_from_desc(tmvp_ctx *ctx, const tmvp_work_desc &d, Job &J, int &mode, int &l)
+{
+    if (d.batch < 0 || (d.flags & ~TMVP_ACCUMULATE)) return TMVP_EARG;
+    const int acc = (d.flags & TMVP_ACCUMULATE) ? 1 : 0;
+    J = base_job(ctx, d.out, d.a, d.s, d.batch);
+    J.accumulate = acc;
+    bool need_aux = false;
+    if (d.op == TMVP_OP_POLYMUL) {
+        if (d.eq_bits < 1 || d.eq_bits > 16 || d.epilogue != 0) return TMVP_EARG;
+        J.w = d.eq_bits;
+        mode = kPoly;
+        l = 1;
+    } else {
+        if (d.param_set < 0 || d.param_set >= TMVP_NUM_PARAM_SETS) return TMVP_EARG;
+        const PS &p = kParams[d.param_set];
+        J.eq = p.eq;
+        J.ep = p.ep;
+        J.et = p.et;
+        l = p.l;
+        if (d.op == TMVP_OP_MATVEC || d.op == TMVP_OP_MATVEC_T) {
+            if (d.epilogue != 0 && d.epilogue != 1) return TMVP_EARG;
+            if (d.epilogue && acc) return TMVP_EARG;
+            J.w = p.eq;
+            if (d.epilogue) {
+                J.epi = kEpiQ2P;
+                J.h = h1_of(p);
+            }
+            mode = d.op == TMVP_OP_MATVEC ? kMatVec : kMatVecT;
+        } else if (d.op == TMVP_OP_INNERPROD) {
+            if (d.epilogue < TMVP_IP_RAW || d.epilogue > TMVP_IP_DEC) return TMVP_EARG;
+            if (d.epilogue != TMVP_IP_RAW && acc) return TMVP_EARG;
+            J.w = p.ep;
+            if (d.epilogue == TMVP_IP_ENC) {
+                J.epi = kEpiEnc;
+                J.h = h1_of(p);
+                J.aux = d.aux;
+                need_aux = true;
+            } else if (d.epilogue == TMVP_IP_DEC) {
+                J.epi = kEpiDec;
+                J.h = h2_of(p);
+                J.aux = d.aux;
+                need_aux = true;
+            }
+            mode = kInner;
+        } else {
+            return TMVP_EARG;
+        }
+    }
+    if (d.batch > 0 && (!d.out || !d.a || !d.s || (need_aux && !d.aux))) return TMVP_EARG;
+    return TMVP_OK;
+}
+
+uint64_t fnv(uint64_t h, uint64_t v)
+{
+    for (int i = 0; i < 8; i++) {
+        h ^= (v >> (8 * i)) & 0xff;
+        h *= 0x100000001b3ull;
+    }
+    return h;
+}
+
+// A checked MFMA work list: the non-empty jobs, their kinds, and the shape
+// signature plans are cached under.
+struct WorkList {
+    Job J[TMVP_MAX_WORK];
+    int kind[TMVP_MAX_WORK];
+    int n;
+    uint64_t sig;
+};
+
+int check_list(tmvp_ctx *ctx, int backend, const tmvp_work_desc *work, int n, WorkList &W, int *modes, int *ls)
+{
+    if (!ctx || n < 0 || n > TMVP_MAX_WORK || (n > 0 && !work)) return TMVP_EARG;
+    W.n = 0;
+    W.sig = 0xcbf29ce484222325ull;
+    for (int i = 0; i < n; i++) {
+        Job J;
+        int mode = 0, l = 0;
+        const int rc = job_from_desc(ctx, work[i], J, mode, l);
+        if (rc) return rc;
+        if (backend == TMVP_BACKEND_MFMA && (J.w < 8 || J.w > 14)) return TMVP_EUNSUPPORTED;
+        if (J.batch == 0) continue;
+        const int k = W.n++;
+        W.J[k] = J;
+        W.kind[k] = mfma_kind(mode, l, J.epi);
+        if (W.kind[k] < 0) return TMVP_EUNSUPPORTED;
+        if (modes) modes[k] = mode;
+        if (ls) ls[k] = l;
+        W.sig = fnv(fnv(W.sig, (uint64_t)W.kind[k]), (uint64_t)J.batch);
+    }
+    if (W.sig == 0) W.sig = 1;
+    return TMVP_OK;
+}
+
+const Plan *find_plan(const tmvp_ctx *ctx, uint64_t sig)
+{
+    for (int i = 0; i < kMaxPlans; i++)
+        if (ctx->plans[i].sig == sig) return &ctx->plans[i];
+    return nullptr;
+}
+
+Plan default_plan(uint64_t sig)
+{
+    Plan P;
+    P.sig = sig;
+    P.wgs_per_cu = 0; // the kernel's full occupancy
+    for (int k = 0; k < kNumKinds; k++) P.weight[k] = 1.0f;
+    return P;
+}
+
+// Splits the resident workgroups between the jobs in proportion to
+// batch x products x weight; each job gets at least one workgroup and no
+// more than it has groups of four instances.
+int build_multi(const tmvp_ctx *ctx, const WorkList &W, const Plan &P, MultiJob &M)
+{
+    int per_cu = mfma_multi_per_cu();
+    if (P.wgs_per_cu >= 1 && P.wgs_per_cu < per_cu) per_cu = P.wgs_per_cu;
+    const double resident = (double)ctx->cus * per_cu;
+    double total = 0.0;
+    double work[TMVP_MAX_WORK];
+    for (int i = 0; i < W.n; i++) {
+        work[i] = (double)W.J[i].batch * mfma_kind_products(W.kind[i]) * P.weight[W.kind[i]];
+        total += work[i];
+    }
+    int begin = 0;
+    M.nseg = W.n;
+    for (int i = 0; i < W.n; i++) {
+        const int64_t cap = (W.J[i].batch + 3) / 4;
+        int64_t b = (int64_t)(resident * work[i] / total + 0.5);
+        if (b < 1) b = 1;
+        if (b > cap) b = cap;
+        M.seg[i].J = W.J[i];
+        M.seg[i].kind = W.kind[i];
+        M.seg[i].block_begin = begin;
+        M.seg[i].blocks = (int)b;
+        begin += (int)b;
+    }
+    return begin;
+}
+
+int launch_list(tmvp_ctx *ctx, const WorkList &W, const Plan &P, hipStream_t st)
+{
+    if (W.n == 0) return TMVP_OK;
+    MultiJob M;
+    const int grid = build_multi(ctx, W, P, M);
+    return hip_status(launch_mfma_multi(M, grid, st));
 }
 
 } // namespace
@@ -201,6 +385,13 @@ int tmvp_init(tmvp_ctx **ctx, int device)
     c->device = device;
     c->d_status = nullptr;
     c->nws = 0;
+    c->ws_clock = 0;
+    c->next_plan = 0;
+    for (int i = 0; i < kMaxPlans; i++) c->plans[i].sig = 0;
+    if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || c->cus < 1) {
+        delete c;
+        return TMVP_EHIP;
+    }
     if (hipMalloc(&c->d_status, sizeof(int)) != hipSuccess) {
         delete c;
         return TMVP_ENOMEM;
@@ -344,6 +535,170 @@ int tmvp_kem_decaps(tmvp_ctx *ctx, int ps, int backend, uint8_t *d_ss, const uin
     if (rc) return rc;
     return hip_status(kem_decaps(P, W, backend == TMVP_BACKEND_MFMA ? 0 : 1, d_ct, d_sk, d_ss, d_fail, batch,
                                  ctx->d_status, reinterpret_cast<hipStream_t>(stream)));
+}
+
+int tmvp_kem_reserve(tmvp_ctx *ctx, int ps, int64_t batch, tmvp_stream_t stream)
+{
+    if (!kem_args_ok(ctx, ps, TMVP_BACKEND_MFMA, batch)) return TMVP_EARG;
+    if (batch == 0) return TMVP_OK;
+    KemParams P;
+    KemWorkspace W;
+    return kem_setup(ctx, ps, batch, stream, P, W);
+}
+
+int tmvp_batch_launch(tmvp_ctx *ctx, int backend, const tmvp_work_desc *work, int n, tmvp_stream_t stream)
+{
+    if (backend != TMVP_BACKEND_MFMA && backend != TMVP_BACKEND_VALU && backend != TMVP_BACKEND_SCHOOLBOOK)
+        return TMVP_EARG;
+    WorkList W;
+    int modes[TMVP_MAX_WORK], ls[TMVP_MAX_WORK];
+    int rc = check_list(ctx, backend, work, n, W, modes, ls);
+    if (rc) return rc;
+    if ((rc = set_device(ctx))) return rc;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (backend != TMVP_BACKEND_MFMA) {
+        for (int i = 0; i < W.n; i++)
+            if ((rc = hip_status(launch_valu(W.J[i], modes[i], ls[i], st)))) return rc;
+        return TMVP_OK;
+    }
+    const Plan *P = find_plan(ctx, W.sig);
+    return launch_list(ctx, W, P ? *P : default_plan(W.sig), st);
+}
+
+int tmvp_batch_autotune(tmvp_ctx *ctx, const tmvp_work_desc *work, int n, tmvp_stream_t stream, float *best_ms)
+{
+    WorkList W;
+    int rc = check_list(ctx, TMVP_BACKEND_MFMA, work, n, W, nullptr, nullptr);
+    if (rc) return rc;
+    for (int i = 0; i < W.n; i++)
+        if (W.J[i].accumulate) return TMVP_EARG;
+    if ((rc = set_device(ctx))) return rc;
+    if (W.n == 0) {
+        if (best_ms) *best_ms = 0.0f;
+        return TMVP_OK;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return TMVP_EHIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return TMVP_EHIP;
+    }
+    // Time per launch of a plan: the best of three runs of five launches.
+    auto measure = [&](const Plan &P, float &ms) -> int {
+        int r;
+        for (int i = 0; i < 2; i++)
+            if ((r = launch_list(ctx, W, P, st))) return r;
+        ms = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            if (hipEventRecord(e0, st) != hipSuccess) return TMVP_EHIP;
+            for (int i = 0; i < 5; i++)
+                if ((r = launch_list(ctx, W, P, st))) return r;
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return TMVP_EHIP;
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, e0, e1) != hipSuccess) return TMVP_EHIP;
+            if (t / 5 < ms) ms = t / 5;
+        }
+        return TMVP_OK;
+    };
+    Plan best = default_plan(W.sig);
+    float best_t = 0.0f;
+    rc = measure(best, best_t);
+    // workgroups per CU
+    for (int w = 1; !rc && w <= mfma_multi_per_cu(); w++) {
+        Plan c = best;
+        c.wgs_per_cu = w;
+        float t = 0.0f;
+        rc = measure(c, t);
+        if (!rc && t < best_t) {
+            best = c;
+            best_t = t;
+        }
+    }
+    // per-kind weights, coordinate descent
+    bool present[kNumKinds] = {};
+    int kinds = 0;
+    for (int i = 0; i < W.n; i++) {
+        kinds += present[W.kind[i]] ? 0 : 1;
+        present[W.kind[i]] = true;
+    }
+    const float mult[4] = {0.7f, 0.85f, 1.2f, 1.4f};
+    for (int pass = 0; pass < 2 && !rc && kinds > 1; pass++)
+        for (int k = 0; k < kNumKinds && !rc; k++) {
+            if (!present[k]) continue;
+            for (int m = 0; m < 4 && !rc; m++) {
+                Plan c = best;
+                c.weight[k] *= mult[m];
+                float t = 0.0f;
+                rc = measure(c, t);
+                if (!rc && t < best_t) {
+                    best = c;
+                    best_t = t;
+                }
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    Plan *slot = nullptr;
+    for (int i = 0; i < kMaxPlans; i++)
+        if (ctx->plans[i].sig == W.sig) slot = &ctx->plans[i];
+    if (!slot) {
+        slot = &ctx->plans[ctx->next_plan];
+        ctx->next_plan = (ctx->next_plan + 1) % kMaxPlans;
+    }
+    *slot = best;
+    if (best_ms) *best_ms = best_t;
+    return TMVP_OK;
+}
+
+int tmvp_graph_begin(tmvp_ctx *ctx, tmvp_stream_t stream)
+{
+    if (!ctx || !stream) return TMVP_EARG;
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    // Resolve the launch-shape queries the kernels make on first use outside the capture.
+    (void)mfma_multi_per_cu();
+    return hip_status(hipStreamBeginCapture(reinterpret_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal));
+}
+
+int tmvp_graph_end(tmvp_ctx *ctx, tmvp_stream_t stream, tmvp_graph **out)
+{
+    if (!ctx || !stream || !out) return TMVP_EARG;
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(reinterpret_cast<hipStream_t>(stream), &g) != hipSuccess || !g) return TMVP_EHIP;
+    hipGraphExec_t x = nullptr;
+    if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return TMVP_EHIP;
+    }
+    tmvp_graph *h = new (std::nothrow) tmvp_graph;
+    if (!h) {
+        (void)hipGraphExecDestroy(x);
+        (void)hipGraphDestroy(g);
+        return TMVP_ENOMEM;
+    }
+    h->graph = g;
+    h->exec = x;
+    *out = h;
+    return TMVP_OK;
+}
+
+int tmvp_graph_launch(tmvp_graph *graph, tmvp_stream_t stream)
+{
+    if (!graph) return TMVP_EARG;
+    return hip_status(hipGraphLaunch(graph->exec, reinterpret_cast<hipStream_t>(stream)));
+}
+
+int tmvp_graph_destroy(tmvp_graph *graph)
+{
+    if (!graph) return TMVP_EARG;
+    int rc = TMVP_OK;
+    if (hipGraphExecDestroy(graph->exec) != hipSuccess) rc = TMVP_EHIP;
+    if (hipGraphDestroy(graph->graph) != hipSuccess) rc = TMVP_EHIP;
+    delete graph;
+    return rc;
 }
 
 } // extern "C"

@@ -63,4 +63,28 @@ __device__ __forceinline__ uint32_t apply_epilogue(const Job &J, uint32_t x, uin
 hipError_t launch_valu(const Job &J, int mode, int l, hipStream_t stream);
 hipError_t launch_mfma(const Job &J, int mode, int l, hipStream_t stream);
 
+// ---- one persistent MFMA launch over several jobs of different shapes ----
+// Kinds enumerate the (L, mode, epilogue) instantiations: 7 product/epilogue
+// combinations x L = 2, 3, 4, then the single polymul.
+constexpr int kNumKinds = 22;
+constexpr int kMaxSegments = 16;
+// Kind of (mode, l, epi), or -1 when there is no such instantiation.
+int mfma_kind(int mode, int l, int epi);
+// Polynomial products per instance of a kind (rows x terms).
+int mfma_kind_products(int kind);
+// A segment: workgroups [block_begin, block_begin + blocks) run job J.
+struct Segment {
+    Job J;
+    int kind;
+    int block_begin;
+    int blocks;
+};
+struct MultiJob {
+    Segment seg[kMaxSegments];
+    int nseg;
+};
+// Workgroups of the multi-job kernel that fit on one CU.
+int mfma_multi_per_cu();
+hipError_t launch_mfma_multi(const MultiJob &M, int grid, hipStream_t stream);
+
 } // namespace tmvp

@@ -111,18 +111,18 @@ __device__ __forceinline__ uint32_t from_next_lane(uint32_t v)
 // 2 waves per SIMD for the four-row FireSaber / Sable-3 shapes.
 template <int L, int MODE> constexpr int kWavesPerEU = (rows_of(MODE, L) * terms_of(MODE, L) > 9) ? 2 : 3;
 
+// LDS of one workgroup (four waves, two slices each).
+template <int L, int MODE> constexpr int kLdsBytes = 4 * 2 * Slice<L, MODE>::BYTES;
+
+// The work of one wave: instances first, first + stride, ... of job J, with
+// the wave's two LDS slices at lds + wave * 2 * Slice::BYTES.
 template <int L, int MODE, int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<L, MODE>))) void mfma_kernel(Job J)
+__device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t stride, uint8_t *lds)
 {
     using S = Slice<L, MODE>;
     constexpr int NR = S::NR, NT = S::NT, NPOLY = S::NPOLY, CJ = S::CJ, ITJ = S::ITJ;
-    // Two slices per wave: term t is staged into slice t & 1 while the term
-    // before it is read from the other one.
-    __shared__ __attribute__((aligned(16))) uint8_t lds[4 * 2 * S::BYTES];
-
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t stride = (int64_t)gridDim.x * 4;
-    int64_t inst = (int64_t)blockIdx.x * 4 + wave;
+    int64_t inst = first;
     if (inst >= J.batch) return; // wave-uniform: each wave owns private LDS slices, no barriers
     uint8_t *const slice0 = lds + wave * 2 * S::BYTES;
     int buf = 0; // slice holding the term about to be read
@@ -329,6 +329,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEU
     if (bad & 0xff00ff00u) *J.status = 1;
 }
 
+template <int L, int MODE, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<L, MODE>))) void mfma_kernel(Job J)
+{
+    // Two slices per wave: term t is staged into slice t & 1 while the term
+    // before it is read from the other one.
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsBytes<L, MODE>];
+    mfma_body<L, MODE, EPI>(J, (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6), (int64_t)gridDim.x * 4, lds);
+}
+
 // Persistent launch: enough workgroups to fill every CU at the occupancy the
 // LDS slice allows, each wave looping over instances with a grid stride.
 template <int L, int MODE, int EPI>
@@ -388,6 +397,86 @@ hipError_t launch_mfma(const Job &J, int mode, int l, hipStream_t stream)
         break;
     }
     return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------- multi-job launch
+
+namespace {
+struct KindInfo {
+    int mode, l, epi;
+};
+constexpr int kCombo[7][2] = {{kMatVec, kEpiMod},  {kMatVec, kEpiQ2P}, {kMatVecT, kEpiMod}, {kMatVecT, kEpiQ2P},
+                              {kInner, kEpiMod},   {kInner, kEpiEnc},  {kInner, kEpiDec}};
+constexpr KindInfo kind_info(int k)
+{
+    return k == 21 ? KindInfo{kPoly, 1, kEpiMod} : KindInfo{kCombo[k / 3][0], 2 + k % 3, kCombo[k / 3][1]};
+}
+// The four-row shapes have the largest slices.
+constexpr int kMultiLds = kLdsBytes<4, kMatVec>;
+static_assert(kMultiLds >= kLdsBytes<3, kMatVec> && kMultiLds >= kLdsBytes<4, kMatVecT> &&
+                  kMultiLds >= kLdsBytes<4, kInner> && kMultiLds >= kLdsBytes<1, kPoly>,
+              "LDS bound");
+
+template <int K> __device__ __forceinline__ void run_kind(const Job &J, int64_t first, int64_t stride, uint8_t *lds)
+{
+    constexpr KindInfo I = kind_info(K);
+    mfma_body<I.l, I.mode, I.epi>(J, first, stride, lds);
+}
+} // namespace
+
+// Workgroup b belongs to the segment whose range holds b; its four waves take
+// instances first, first + stride, ... of that segment's job.  Two waves per
+// SIMD: the register budget of the largest shape.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mfma_multi_kernel(MultiJob M)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kMultiLds];
+    int k = 0;
+    while (k + 1 < M.nseg && (int)blockIdx.x >= M.seg[k + 1].block_begin) k++;
+    const Segment &G = M.seg[k];
+    const int64_t first = (int64_t)((int)blockIdx.x - G.block_begin) * 4 + (threadIdx.x >> 6);
+    const int64_t stride = (int64_t)G.blocks * 4;
+    switch (G.kind) {
+#define TMVP_KIND(K)                                                                                                 \
+    case K: run_kind<K>(G.J, first, stride, lds); break;
+        TMVP_KIND(0) TMVP_KIND(1) TMVP_KIND(2) TMVP_KIND(3) TMVP_KIND(4) TMVP_KIND(5) TMVP_KIND(6) TMVP_KIND(7)
+        TMVP_KIND(8) TMVP_KIND(9) TMVP_KIND(10) TMVP_KIND(11) TMVP_KIND(12) TMVP_KIND(13) TMVP_KIND(14)
+        TMVP_KIND(15) TMVP_KIND(16) TMVP_KIND(17) TMVP_KIND(18) TMVP_KIND(19) TMVP_KIND(20) TMVP_KIND(21)
+#undef TMVP_KIND
+    default: break;
+    }
+}
+
+int mfma_kind(int mode, int l, int epi)
+{
+    for (int k = 0; k < kNumKinds; k++) {
+        const KindInfo I = kind_info(k);
+        if (I.mode == mode && I.epi == epi && (mode == kPoly || I.l == l)) return k;
+    }
+    return -1;
+}
+
+int mfma_kind_products(int kind)
+{
+    const KindInfo I = kind_info(kind);
+    return rows_of(I.mode, I.l) * terms_of(I.mode, I.l);
+}
+
+int mfma_multi_per_cu()
+{
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mfma_multi_kernel, 256, 0) != hipSuccess || n < 1) n = 1;
+        per_cu = n;
+    }
+    return per_cu;
+}
+
+hipError_t launch_mfma_multi(const MultiJob &M, int grid, hipStream_t stream)
+{
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mfma_multi_kernel, dim3((unsigned)grid), dim3(256), 0, stream, M);
+    return hipGetLastError();
 }
 
 } // namespace tmvp

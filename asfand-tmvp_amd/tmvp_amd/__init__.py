@@ -66,7 +66,19 @@ EXPORTS = ("tmvp_init", "tmvp_destroy", "tmvp_ctx_status", "tmvp_strerror", "tmv
            "tmvp_version", "tmvp_polymul", "tmvp_matvec", "tmvp_innerprod",
            "tmvp_kem_keypair", "tmvp_kem_encaps", "tmvp_kem_decaps",
            "tmvp_host_configure", "tmvp_host_set_randomness", "tmvp_host_status", "tmvp_host_release",
-           "saber_enc", "saber_dec", "SABLE_enc", "SABLE_dec")
+           "saber_enc", "saber_dec", "SABLE_enc", "SABLE_dec",
+           "tmvp_batch_launch", "tmvp_batch_autotune", "tmvp_graph_begin", "tmvp_graph_end", "tmvp_graph_launch",
+           "tmvp_graph_destroy", "tmvp_kem_reserve")
+
+OP_POLYMUL, OP_MATVEC, OP_MATVEC_T, OP_INNERPROD = 0, 1, 2, 3
+MAX_WORK = 16
+
+
+class WorkDesc(ctypes.Structure):
+    """tmvp_work_desc: one job of a mixed batch launch."""
+    _fields_ = [("op", ctypes.c_int), ("param_set", ctypes.c_int), ("epilogue", ctypes.c_int),
+                ("eq_bits", ctypes.c_int), ("flags", ctypes.c_int), ("batch", ctypes.c_int64),
+                ("out", ctypes.c_void_p), ("a", ctypes.c_void_p), ("s", ctypes.c_void_p), ("aux", ctypes.c_void_p)]
 
 
 def load_library(path=LIB_PATH):
@@ -99,6 +111,13 @@ def load_library(path=LIB_PATH):
         "saber_dec": (None, [ctypes.c_uint8, P, P, P, P]),
         "SABLE_enc": (None, [P, P, P, i]),
         "SABLE_dec": (None, [P, P, P, P, i]),
+        "tmvp_batch_launch": (i, [P, i, P, i, P]),
+        "tmvp_batch_autotune": (i, [P, P, i, P, ctypes.POINTER(ctypes.c_float)]),
+        "tmvp_graph_begin": (i, [P, P]),
+        "tmvp_graph_end": (i, [P, P, ctypes.POINTER(P)]),
+        "tmvp_graph_launch": (i, [P, P]),
+        "tmvp_graph_destroy": (i, [P]),
+        "tmvp_kem_reserve": (i, [P, i, i64, P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -335,6 +354,67 @@ class Context:
                                             _ptr(fail), int(batch), _stream(stream))
         if rc:
             raise TmvpError(rc, "tmvp_kem_decaps")
+
+
+    def batch_launch(self, descs, backend=MFMA, stream=None):
+        arr = (WorkDesc * max(1, len(descs)))(*descs)
+        rc = load_library().tmvp_batch_launch(self.handle, backend, ctypes.cast(arr, ctypes.c_void_p), len(descs),
+                                              _stream(stream))
+        if rc:
+            raise TmvpError(rc, "tmvp_batch_launch")
+
+    def batch_autotune(self, descs, stream=None):
+        arr = (WorkDesc * max(1, len(descs)))(*descs)
+        ms = ctypes.c_float()
+        rc = load_library().tmvp_batch_autotune(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(descs),
+                                                _stream(stream), ctypes.byref(ms))
+        if rc:
+            raise TmvpError(rc, "tmvp_batch_autotune")
+        return ms.value
+
+    def kem_reserve(self, param_set, batch, stream=None):
+        ps = PARAM_SET_NAMES.get(param_set, param_set)
+        rc = load_library().tmvp_kem_reserve(self.handle, int(ps), int(batch), _stream(stream))
+        if rc:
+            raise TmvpError(rc, "tmvp_kem_reserve")
+
+    def capture(self, stream, fn):
+        """Record fn() (library calls on `stream`) into a Graph."""
+        lib = load_library()
+        rc = lib.tmvp_graph_begin(self.handle, _stream(stream))
+        if rc:
+            raise TmvpError(rc, "tmvp_graph_begin")
+        try:
+            fn()
+        finally:
+            g = ctypes.c_void_p()
+            rc = lib.tmvp_graph_end(self.handle, _stream(stream), ctypes.byref(g))
+        if rc:
+            raise TmvpError(rc, "tmvp_graph_end")
+        return Graph(g)
+
+
+class Graph:
+    """An instantiated hipGraph of library calls (tmvp_graph)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self, stream=None):
+        rc = load_library().tmvp_graph_launch(self.handle, _stream(stream))
+        if rc:
+            raise TmvpError(rc, "tmvp_graph_launch")
+
+    def destroy(self):
+        if self.handle is not None and self.handle.value:
+            load_library().tmvp_graph_destroy(self.handle)
+        self.handle = None
+
+
+def work(op, out, a, s, batch, param_set=SABER, epilogue=0, aux=None, eq_bits=0, flags=0):
+    """A WorkDesc over DeviceBuffers (or raw device addresses)."""
+    return WorkDesc(op, int(PARAM_SET_NAMES.get(param_set, param_set)), epilogue, eq_bits, flags, int(batch),
+                    _ptr(out), _ptr(a), _ptr(s), _ptr(aux))
 
 
 # ---- numpy conveniences (allocate, copy, run, copy back, synchronise) ----

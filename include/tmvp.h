@@ -135,6 +135,60 @@ int tmvp_innerprod(tmvp_ctx *ctx, int param_set, int backend, int epilogue, uint
                    const uint16_t *d_b, const uint16_t *d_s, const uint16_t *d_aux,
                    int64_t batch, int flags, tmvp_stream_t stream);
 
+/* ---- mixed batches: one persistent launch over many jobs ----
+ * The reference launches one kernel per product and parameter set with one
+ * block per instance (S/src/cuda_kernel.cu:129, :162; SB/src/SABLE_indcpa.cu:186,
+ * :217).  A work list instead runs any mix of products, parameter sets and
+ * epilogues in ONE launch of a persistent MFMA kernel: its workgroups are
+ * split between the jobs in proportion to their products, with the split and
+ * the workgroups per CU taken from a plan that tmvp_batch_autotune() measures
+ * for the list's shape (kinds and batch sizes); without one, the split is by
+ * product count at the kernel's full occupancy.  The VALU backends issue one
+ * launch per job.  Each descriptor has the meaning of the matching single
+ * call: tmvp_polymul (eq_bits), tmvp_matvec (epilogue 0 = mod q, 1 = round to
+ * p) or tmvp_innerprod (epilogue = tmvp_ip_epilogue). */
+enum tmvp_op {
+    TMVP_OP_POLYMUL = 0,
+    TMVP_OP_MATVEC = 1,
+    TMVP_OP_MATVEC_T = 2,
+    TMVP_OP_INNERPROD = 3
+};
+#define TMVP_MAX_WORK 16
+
+typedef struct {
+    int op;        /* tmvp_op */
+    int param_set; /* not read for TMVP_OP_POLYMUL */
+    int epilogue;
+    int eq_bits;   /* TMVP_OP_POLYMUL only */
+    int flags;     /* TMVP_ACCUMULATE (raw outputs only) */
+    int64_t batch;
+    uint16_t *out;
+    const uint16_t *a; /* A, b or a */
+    const uint16_t *s;
+    const uint16_t *aux; /* InnerProd ENC / DEC */
+} tmvp_work_desc;
+
+/* Every descriptor is checked before anything is launched. */
+int tmvp_batch_launch(tmvp_ctx *ctx, int backend, const tmvp_work_desc *work, int n, tmvp_stream_t stream);
+
+/* Times candidate plans for this work list on its own buffers (outputs are
+ * overwritten; TMVP_ACCUMULATE is refused), keeps the fastest in the context
+ * for later launches of lists of the same shape, and returns its time per
+ * launch in *best_ms (may be NULL).  Synchronises `stream`. */
+int tmvp_batch_autotune(tmvp_ctx *ctx, const tmvp_work_desc *work, int n, tmvp_stream_t stream, float *best_ms);
+
+/* ---- hipGraph capture of a sequence of library calls on one stream ----
+ * Between tmvp_graph_begin and tmvp_graph_end every tmvp call on `stream` is
+ * recorded instead of run; the graph replays it with the same buffers.  A
+ * KEM call whose workspace would have to grow fails with TMVP_EUNSUPPORTED
+ * during capture: reserve it first with tmvp_kem_reserve.  The legacy NULL
+ * stream cannot be captured. */
+typedef struct tmvp_graph tmvp_graph;
+int tmvp_graph_begin(tmvp_ctx *ctx, tmvp_stream_t stream);
+int tmvp_graph_end(tmvp_ctx *ctx, tmvp_stream_t stream, tmvp_graph **graph);
+int tmvp_graph_launch(tmvp_graph *graph, tmvp_stream_t stream);
+int tmvp_graph_destroy(tmvp_graph *graph);
+
 /* ---- batched KEM (fused GPU pipeline: SHAKE-128 GenMatrix, CBD GenSecret,
  * MatrixVectorMul, InnerProd, rounding, codecs, Fujisaki-Okamoto hashes) ----
  * Byte layouts are the reference's: pk = pack_EP(b) | seed_A,
@@ -164,6 +218,10 @@ int tmvp_kem_encaps(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_ct, ui
  * rejected ciphertext. */
 int tmvp_kem_decaps(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_ss, const uint8_t *d_ct,
                     const uint8_t *d_sk, uint8_t *d_fail, int64_t batch, tmvp_stream_t stream);
+
+/* Grow the KEM workspace of `stream` to `batch` instances of `param_set` now
+ * (synchronises the device), so later KEM calls on it allocate nothing. */
+int tmvp_kem_reserve(tmvp_ctx *ctx, int param_set, int64_t batch, tmvp_stream_t stream);
 
 /* ---- the reference's host entry points (same names and signatures) ----
  * saber_enc / saber_dec (S/include/cuda_kernel.cuh:6-7) and SABLE_enc /
