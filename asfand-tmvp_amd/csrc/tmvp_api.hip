@@ -166,6 +166,18 @@ int kem_setup(tmvp_ctx *ctx, int ps, int64_t batch, tmvp_stream_t stream, KemPar
     return TMVP_OK;
 }
 
+} // namespace
+
+namespace tmvp {
+KemParams kem_params_of(int ps) { return kem_params(ps); }
+int kem_backend_ok(int backend)
+{
+    return backend == TMVP_BACKEND_MFMA || backend == TMVP_BACKEND_VALU || backend == TMVP_BACKEND_SCHOOLBOOK;
+}
+} // namespace tmvp
+
+namespace {
+
 Job base_job(tmvp_ctx *ctx, uint16_t *out, const uint16_t *a, const uint16_t *s, int64_t batch)
 {
     Job J;

@@ -21,6 +21,15 @@ struct KemParams {
 struct KemWorkspace {
     uint16_t *A, *s, *v1, *v2, *bits, *poly;
     uint8_t *buf, *kr, *ctp, *araw;
+    uint8_t *pkg, *skg, *hpkg; // key material gathered from a key store
+};
+
+// Device-resident keys of one parameter set: pk [n][pk], sk [n][sk] (may be
+// null), the matrices A [n][L][L][256] and H(pk) [n][32] derived from them.
+struct KeyStore {
+    const uint8_t *pk, *sk;
+    const uint16_t *A;
+    const uint8_t *hpk;
 };
 
 size_t kem_workspace_bytes(const KemParams &P, int64_t B);
@@ -35,5 +44,17 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
 // fail (optional): [B] bytes, 1 where the ciphertext was rejected.
 hipError_t kem_decaps(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *ct, const uint8_t *sk,
                       uint8_t *ss, uint8_t *fail, int64_t B, int *status, hipStream_t st);
+
+// Encapsulation / decapsulation for instance t under key keys[t] of K
+// (ids checked by the caller); A and H(pk) come from the store.
+hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backend, const KeyStore &K,
+                            const int32_t *keys, const uint8_t *rnd, uint8_t *ct, uint8_t *ss, int64_t B, int *status,
+                            hipStream_t st);
+hipError_t kem_decaps_keyed(const KemParams &P, const KemWorkspace &W, int backend, const KeyStore &K,
+                            const int32_t *keys, const uint8_t *ct, uint8_t *ss, uint8_t *fail, int64_t B,
+                            int *status, hipStream_t st);
+// A and H(pk) of n public keys; araw is scratch of n * P.araw bytes.
+hipError_t kem_store_derive(const KemParams &P, const uint8_t *pk, int64_t n, uint16_t *A, uint8_t *hpk,
+                            uint8_t *araw, hipStream_t st);
 
 } // namespace tmvp

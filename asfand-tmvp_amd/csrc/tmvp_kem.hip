@@ -138,12 +138,13 @@ __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t
 }
 
 // Encapsulation: A from the pk tail; m = H(rnd), kr = G(m | H(pk)), s' =
-// GenSecret(kr[32:64]).
+// GenSecret(kr[32:64]).  Thread t takes role t + t0: t0 = B skips the matrix
+// (already resident), and hpk (if not null) holds H(pk) per instance.
 __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
-                                                       uint8_t *buf, uint8_t *kr, uint8_t *araw, uint16_t *S,
-                                                       int64_t B)
+                                                       const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint8_t *araw,
+                                                       uint16_t *S, int64_t B, int64_t t0)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x + t0;
     if (t >= 2 * B) return;
     uint64_t m[4];
     if (t < B) {
@@ -154,7 +155,8 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
         uint64_t r[4], h[4], coins[4];
         load32(r, rnd + i * 32);
         sha3_256_32(m, r);
-        sha3_256(h, pk + i * P.pk, P.pk);
+        if (hpk) load32(h, hpk + i * 32);
+        else sha3_256(h, pk + i * P.pk, P.pk);
         hash_g(m, h, buf + i * 64, kr + i * 64, coins);
         gen_secret(P, coins, S + i * (P.l * kN));
     }
@@ -164,9 +166,9 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
 // its bits, kr = G(m' | H(pk)) with H(pk) from the sk, s' = GenSecret(kr[32:64]).
 __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
                                                        uint8_t *buf, uint8_t *kr, uint8_t *araw, uint16_t *S,
-                                                       int64_t B)
+                                                       int64_t B, int64_t t0)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x + t0;
     if (t >= 2 * B) return;
     uint64_t m[4];
     if (t < B) {
@@ -276,7 +278,39 @@ __global__ __launch_bounds__(256) void copy_kernel(const uint8_t *src, int64_t s
         reinterpret_cast<const uint64_t *>(src + t * src_stride + src_off)[k];
 }
 
+// dst[t] = src[keys[t]] for `words` 8-byte words per row (key ids checked by the caller).
+__global__ __launch_bounds__(256) void gather_kernel(const uint8_t *src, int64_t src_stride, const int32_t *keys,
+                                                     uint8_t *dst, int64_t dst_stride, int words, int64_t B)
+{
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= B * words) return;
+    const int64_t t = gid / words;
+    const int k = (int)(gid - t * words);
+    reinterpret_cast<uint64_t *>(dst + t * dst_stride)[k] =
+        reinterpret_cast<const uint64_t *>(src + (int64_t)keys[t] * src_stride)[k];
+}
+
 // ---------------------------------------------------------------- hashing steps (one thread per instance)
+
+// H(pk) of each public key (for a key store).
+__global__ __launch_bounds__(64) void hash_pk_kernel(KemParams P, const uint8_t *pk, uint8_t *hpk, int64_t B)
+{
+    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (t >= B) return;
+    uint64_t h[4];
+    sha3_256(h, pk + t * P.pk, P.pk);
+    store32(hpk + t * 32, h);
+}
+
+// The GenMatrix stream of each public key's seed (for a key store).
+__global__ __launch_bounds__(64) void matrix_stream_kernel(KemParams P, const uint8_t *pk, uint8_t *araw, int64_t B)
+{
+    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (t >= B) return;
+    uint64_t m[4];
+    load32(m, pk + t * P.pk + P.pvc);
+    gen_matrix_stream(P, m, araw + t * P.araw);
+}
 
 // Key generation, last step: H(pk) into the sk.
 __global__ __launch_bounds__(64) void kp_hash_kernel(KemParams P, const uint8_t *pk, uint8_t *sk, int64_t B)
@@ -459,8 +493,8 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
                       uint8_t *ct, uint8_t *ss, int64_t B, int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(enc_front_kernel, dim3(blocks_for(2 * B, 64)), dim3(64), 0, st, P, rnd, pk, W.buf, W.kr,
-                       W.araw, W.s, B);
+    hipLaunchKernelGGL(enc_front_kernel, dim3(blocks_for(2 * B, 64)), dim3(64), 0, st, P, rnd, pk, nullptr, W.buf,
+                       W.kr, W.araw, W.s, B, (int64_t)0);
     TRY(hipGetLastError());
     TRY(unpack(W.araw, P.araw, 0, W.A, P.l * P.l * kN, P.eq, P.l * P.l * kN, kXfNone, B, st));
     TRY(indcpa_enc_tail(P, W, backend, pk, P.pk, ct, B, status, st));
@@ -468,10 +502,12 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
     return hipGetLastError();
 }
 
-hipError_t kem_decaps(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *ct, const uint8_t *sk,
-                      uint8_t *ss, uint8_t *fail, int64_t B, int *status, hipStream_t st)
+// Decapsulation; with `resident` set, the matrix A of every instance is
+// already in W.A and is not generated again.
+static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *ct,
+                               const uint8_t *sk, uint8_t *ss, uint8_t *fail, int64_t B, int *status, hipStream_t st,
+                               bool resident)
 {
-    if (B <= 0) return hipSuccess;
     const int L = P.l;
     // IND-CPA decryption (S/src/cuda_kernel.cu:341-370; SB/src/SABLE_indcpa.cu:422-463)
     if (P.sable)
@@ -482,12 +518,70 @@ hipError_t kem_decaps(const KemParams &P, const KemWorkspace &W, int backend, co
     TRY(unpack(ct, P.ct, P.pvc, W.bits, kN, P.et, kN, kXfNone, B, st));
     TRY(product(P, backend, kInner, kEpiDec, W.poly, W.v2, W.s, W.bits, B, status, st));
     // re-encryption of m' under the pk stored in the sk
-    hipLaunchKernelGGL(dec_front_kernel, dim3(blocks_for(2 * B, 64)), dim3(64), 0, st, P, W.poly, sk, W.buf, W.kr,
-                       W.araw, W.s, B);
+    const int64_t t0 = resident ? B : 0;
+    hipLaunchKernelGGL(dec_front_kernel, dim3(blocks_for(2 * B - t0, 64)), dim3(64), 0, st, P, W.poly, sk, W.buf,
+                       W.kr, W.araw, W.s, B, t0);
     TRY(hipGetLastError());
-    TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
+    if (!resident) TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
     TRY(indcpa_enc_tail(P, W, backend, sk + P.indcpa_sk, P.sk, W.ctp, B, status, st));
     hipLaunchKernelGGL(dec_fin_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, ct, W.ctp, sk, W.kr, ss, fail, B);
+    return hipGetLastError();
+}
+
+hipError_t kem_decaps(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *ct, const uint8_t *sk,
+                      uint8_t *ss, uint8_t *fail, int64_t B, int *status, hipStream_t st)
+{
+    if (B <= 0) return hipSuccess;
+    return decaps_chain(P, W, backend, ct, sk, ss, fail, B, status, st, false);
+}
+
+static hipError_t gather(const uint8_t *src, int64_t src_stride, const int32_t *keys, uint8_t *dst, int64_t dst_stride,
+                         int bytes, int64_t B, hipStream_t st)
+{
+    const int words = bytes / 8;
+    hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(B * words, 256)), dim3(256), 0, st, src, src_stride, keys, dst,
+                       dst_stride, words, B);
+    return hipGetLastError();
+}
+
+hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backend, const KeyStore &K,
+                            const int32_t *keys, const uint8_t *rnd, uint8_t *ct, uint8_t *ss, int64_t B, int *status,
+                            hipStream_t st)
+{
+    if (B <= 0) return hipSuccess;
+    const int abytes = P.l * P.l * kN * 2;
+    TRY(gather(K.pk, P.pk, keys, W.pkg, P.pk, P.pk, B, st));
+    TRY(gather(reinterpret_cast<const uint8_t *>(K.A), abytes, keys, reinterpret_cast<uint8_t *>(W.A), abytes, abytes,
+               B, st));
+    TRY(gather(K.hpk, 32, keys, W.hpkg, 32, 32, B, st));
+    hipLaunchKernelGGL(enc_front_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg, W.buf, W.kr,
+                       W.araw, W.s, B, B);
+    TRY(hipGetLastError());
+    TRY(indcpa_enc_tail(P, W, backend, W.pkg, P.pk, ct, B, status, st));
+    hipLaunchKernelGGL(enc_hash2_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, ct, W.kr, ss, B);
+    return hipGetLastError();
+}
+
+hipError_t kem_decaps_keyed(const KemParams &P, const KemWorkspace &W, int backend, const KeyStore &K,
+                            const int32_t *keys, const uint8_t *ct, uint8_t *ss, uint8_t *fail, int64_t B,
+                            int *status, hipStream_t st)
+{
+    if (B <= 0) return hipSuccess;
+    const int abytes = P.l * P.l * kN * 2;
+    TRY(gather(K.sk, P.sk, keys, W.skg, P.sk, P.sk, B, st));
+    TRY(gather(reinterpret_cast<const uint8_t *>(K.A), abytes, keys, reinterpret_cast<uint8_t *>(W.A), abytes, abytes,
+               B, st));
+    return decaps_chain(P, W, backend, ct, W.skg, ss, fail, B, status, st, true);
+}
+
+hipError_t kem_store_derive(const KemParams &P, const uint8_t *pk, int64_t n, uint16_t *A, uint8_t *hpk,
+                            uint8_t *araw, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(matrix_stream_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, st, P, pk, araw, n);
+    TRY(hipGetLastError());
+    TRY(unpack(araw, P.araw, 0, A, P.l * P.l * kN, P.eq, P.l * P.l * kN, kXfNone, n, st));
+    hipLaunchKernelGGL(hash_pk_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, st, P, pk, hpk, n);
     return hipGetLastError();
 }
 
@@ -505,6 +599,9 @@ size_t kem_workspace_bytes(const KemParams &P, int64_t B)
     b += n * 64;             // kr
     b += n * (size_t)P.ct;   // ctp
     b += n * (size_t)P.araw; // araw
+    b += n * (size_t)P.pk;   // pkg
+    b += n * (size_t)P.sk;   // skg
+    b += n * 32;             // hpkg
     return b + 16 * 256;
 }
 
@@ -526,6 +623,9 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
     W.kr = take(n * 64);
     W.ctp = take(n * (size_t)P.ct);
     W.araw = take(n * (size_t)P.araw);
+    W.pkg = take(n * (size_t)P.pk);
+    W.skg = take(n * (size_t)P.sk);
+    W.hpkg = take(n * 32);
 }
 
 } // namespace tmvp

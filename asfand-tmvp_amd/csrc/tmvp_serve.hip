@@ -1,0 +1,497 @@
+// Serving loop: continuous batching of KEM encapsulation / decapsulation
+// requests over device-resident keys.  The reference has no serving layer
+// (its host functions run one fixed batch per call, S/src/cuda_kernel.cu:25,
+// SB/src/SABLE_indcpa.cu:24); this is the request path around the same
+// kernels.
+//
+// Design:
+//  - Key store per parameter set, in HBM: pk, sk, and the matrix A and H(pk)
+//    derived once per key, so a request never runs the GenMatrix sponge.
+//  - Slots: each slot is a stream with its own workspace, device and pinned
+//    host staging, sized for max_batch instances of the largest parameter
+//    set.  The HBM budget fixes how many slots exist; a batch is admitted when
+//    a slot is free, so batches of different parameter sets and operations
+//    run concurrently on the device.
+//  - A scheduler thread forms each batch from the queue (parameter set,
+//    operation) whose oldest request waited longest, taking requests in
+//    arrival order up to max_batch instances, and retires finished slots by
+//    polling their events.  Requests arriving meanwhile join the next batch
+//    that a free slot takes.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/tmvp.h"
+#include "tmvp_internal.h"
+#include "tmvp_kem.h"
+
+using namespace tmvp;
+
+namespace tmvp {
+KemParams kem_params_of(int ps); // tmvp_api.hip
+int kem_backend_ok(int backend);
+} // namespace tmvp
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Store {
+    uint8_t *pk = nullptr, *sk = nullptr, *hpk = nullptr;
+    uint16_t *A = nullptr;
+    bool has_sk = true;
+    int64_t n = 0, cap = 0;
+};
+
+struct Pending {
+    uint64_t id;
+    tmvp_srv_request req;
+    double t_submit;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    uint8_t *ws = nullptr;    // KEM workspace
+    uint8_t *d_in = nullptr;  // keys | inputs
+    uint8_t *d_out = nullptr; // ct | ss | fail
+    int *d_status = nullptr;
+    uint8_t *h_in = nullptr, *h_out = nullptr; // pinned staging
+    int *h_status = nullptr;
+    bool busy = false;
+    int op = 0, ps = 0;
+    int64_t instances = 0;
+    double t_start = 0;
+    std::vector<Pending> reqs;
+};
+
+} // namespace
+
+struct tmvp_server {
+    tmvp_ctx *ctx;
+    int device, backend;
+    int64_t max_batch;
+    size_t ws_bytes, in_bytes, out_bytes;
+    Clock::time_point t0;
+    Store store[TMVP_NUM_PARAM_SETS];
+    std::vector<Slot> slots;
+    std::deque<Pending> queue[TMVP_NUM_PARAM_SETS][2];
+    std::deque<tmvp_srv_completion> done;
+    std::mutex mu;
+    std::condition_variable wake, completed;
+    std::thread worker;
+    bool stop = false;
+    uint64_t next_id = 1;
+    int64_t pending_requests = 0, inflight = 0;
+    tmvp_srv_stats stats{};
+    int fatal = TMVP_OK;
+
+    double now_us() const { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+// Largest per-instance sizes over all parameter sets.
+void max_sizes(int64_t B, size_t &ws, size_t &in, size_t &out)
+{
+    ws = in = out = 0;
+    for (int ps = 0; ps < TMVP_NUM_PARAM_SETS; ps++) {
+        const KemParams P = kem_params_of(ps);
+        const size_t w = kem_workspace_bytes(P, B);
+        const size_t i = up((size_t)B * 4) + up((size_t)B * (size_t)(P.ct > 32 ? P.ct : 32));
+        const size_t o = (size_t)B * (P.ct + 32 + 1);
+        ws = w > ws ? w : ws;
+        in = i > in ? i : in;
+        out = o > out ? o : out;
+    }
+}
+
+void free_slot(Slot &s)
+{
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    if (s.ws) (void)hipFree(s.ws);
+    if (s.d_in) (void)hipFree(s.d_in);
+    if (s.d_out) (void)hipFree(s.d_out);
+    if (s.d_status) (void)hipFree(s.d_status);
+    if (s.h_in) (void)hipHostFree(s.h_in);
+    if (s.h_out) (void)hipHostFree(s.h_out);
+    if (s.h_status) (void)hipHostFree(s.h_status);
+    s = Slot();
+}
+
+int alloc_slot(tmvp_server *S, Slot &s)
+{
+    if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return TMVP_EHIP;
+    if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) return TMVP_EHIP;
+    if (hipMalloc(&s.ws, S->ws_bytes) != hipSuccess || hipMalloc(&s.d_in, S->in_bytes) != hipSuccess ||
+        hipMalloc(&s.d_out, S->out_bytes) != hipSuccess || hipMalloc(&s.d_status, sizeof(int)) != hipSuccess)
+        return TMVP_ENOMEM;
+    if (hipHostMalloc(&s.h_in, S->in_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&s.h_out, S->out_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&s.h_status, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        return TMVP_ENOMEM;
+    if (hipMemset(s.d_status, 0, sizeof(int)) != hipSuccess) return TMVP_EHIP;
+    return TMVP_OK;
+}
+
+// Launch the requests in s.reqs on slot s (lock held).
+int launch_slot(tmvp_server *S, Slot &s)
+{
+    const KemParams P = kem_params_of(s.ps);
+    const int64_t B = s.instances;
+    // pinned staging: keys, then inputs
+    int32_t *hkeys = reinterpret_cast<int32_t *>(s.h_in);
+    uint8_t *hin = s.h_in + up((size_t)B * 4);
+    const size_t in_each = s.op == TMVP_SRV_ENCAPS ? 32 : (size_t)P.ct;
+    int64_t off = 0;
+    for (const Pending &p : s.reqs) {
+        memcpy(hkeys + off, p.req.keys, (size_t)p.req.count * 4);
+        memcpy(hin + off * in_each, p.req.in, (size_t)p.req.count * in_each);
+        off += p.req.count;
+    }
+    const size_t in_total = up((size_t)B * 4) + (size_t)B * in_each;
+    if (hipMemcpyAsync(s.d_in, s.h_in, in_total, hipMemcpyHostToDevice, s.stream) != hipSuccess) return TMVP_EHIP;
+    KemWorkspace W;
+    kem_workspace_carve(P, B, s.ws, W);
+    const Store &st = S->store[s.ps];
+    KeyStore K{st.pk, st.sk, st.A, st.hpk};
+    const int32_t *dkeys = reinterpret_cast<const int32_t *>(s.d_in);
+    const uint8_t *din = s.d_in + up((size_t)B * 4);
+    // outputs, packed: encapsulation ct | ss, decapsulation ss | fail
+    const size_t out_total = s.op == TMVP_SRV_ENCAPS ? (size_t)B * (P.ct + 32) : (size_t)B * 33;
+    const int backend = S->backend == TMVP_BACKEND_MFMA ? 0 : 1;
+    hipError_t e;
+    if (s.op == TMVP_SRV_ENCAPS)
+        e = kem_encaps_keyed(P, W, backend, K, dkeys, din, s.d_out, s.d_out + (size_t)B * P.ct, B, s.d_status,
+                             s.stream);
+    else
+        e = kem_decaps_keyed(P, W, backend, K, dkeys, din, s.d_out, s.d_out + (size_t)B * 32, B, s.d_status,
+                             s.stream);
+    if (e != hipSuccess) return TMVP_EHIP;
+    if (hipMemcpyAsync(s.h_out, s.d_out, out_total, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+        hipMemcpyAsync(s.h_status, s.d_status, sizeof(int), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+        hipEventRecord(s.done, s.stream) != hipSuccess)
+        return TMVP_EHIP;
+    return TMVP_OK;
+}
+
+// Copy a finished slot's results to the requests' buffers (lock held).
+void retire_slot(tmvp_server *S, Slot &s, int rc)
+{
+    const KemParams P = kem_params_of(s.ps);
+    const int64_t B = s.instances;
+    if (!rc && *s.h_status) {
+        rc = TMVP_ERANGE;
+        *s.h_status = 0;
+        (void)hipMemsetAsync(s.d_status, 0, sizeof(int), s.stream);
+    }
+    const bool enc = s.op == TMVP_SRV_ENCAPS;
+    const uint8_t *hct = s.h_out, *hss = enc ? s.h_out + (size_t)B * P.ct : s.h_out, *hfail = s.h_out + (size_t)B * 32;
+    const double t = S->now_us();
+    int64_t off = 0;
+    for (const Pending &p : s.reqs) {
+        const int64_t n = p.req.count;
+        if (!rc) {
+            if (s.op == TMVP_SRV_ENCAPS) memcpy(p.req.ct, hct + off * P.ct, (size_t)n * P.ct);
+            memcpy(p.req.ss, hss + off * 32, (size_t)n * 32);
+            if (s.op == TMVP_SRV_DECAPS && p.req.fail) memcpy(p.req.fail, hfail + off, (size_t)n);
+        }
+        tmvp_srv_completion c;
+        c.id = p.id;
+        c.tag = p.req.tag;
+        c.status = rc;
+        c.batch_requests = (int)s.reqs.size();
+        c.batch_instances = B;
+        c.submit_us = p.t_submit;
+        c.start_us = s.t_start;
+        c.done_us = t;
+        S->done.push_back(c);
+        off += n;
+    }
+    S->stats.requests_done += (int64_t)s.reqs.size();
+    S->stats.instances_done += B;
+    S->stats.batches += 1;
+    S->inflight -= 1;
+    s.busy = false;
+    s.reqs.clear();
+    S->completed.notify_all();
+}
+
+// Form the next batch: the (set, op) queue whose head waited longest (lock held).
+bool pick_batch(tmvp_server *S, Slot &s)
+{
+    int bps = -1, bop = -1;
+    double oldest = 0;
+    for (int ps = 0; ps < TMVP_NUM_PARAM_SETS; ps++)
+        for (int op = 0; op < 2; op++) {
+            const auto &q = S->queue[ps][op];
+            if (!q.empty() && (bps < 0 || q.front().t_submit < oldest)) {
+                bps = ps;
+                bop = op;
+                oldest = q.front().t_submit;
+            }
+        }
+    if (bps < 0) return false;
+    auto &q = S->queue[bps][bop];
+    s.ps = bps;
+    s.op = bop;
+    s.instances = 0;
+    s.reqs.clear();
+    while (!q.empty() && s.instances + q.front().req.count <= S->max_batch) {
+        s.instances += q.front().req.count;
+        s.reqs.push_back(q.front());
+        q.pop_front();
+    }
+    S->pending_requests -= (int64_t)s.reqs.size();
+    return true;
+}
+
+void worker_main(tmvp_server *S)
+{
+    (void)hipSetDevice(S->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    while (!S->stop) {
+        bool progress = false;
+        for (Slot &s : S->slots) {
+            if (!s.busy) continue;
+            const hipError_t q = hipEventQuery(s.done);
+            if (q == hipErrorNotReady) continue;
+            retire_slot(S, s, q == hipSuccess ? TMVP_OK : TMVP_EHIP);
+            progress = true;
+        }
+        for (Slot &s : S->slots) {
+            if (s.busy || S->pending_requests == 0) continue;
+            if (!pick_batch(S, s)) break;
+            s.busy = true;
+            s.t_start = S->now_us();
+            S->inflight += 1;
+            if (S->inflight > S->stats.max_concurrent) S->stats.max_concurrent = S->inflight;
+            const int rc = launch_slot(S, s);
+            if (rc) {
+                (void)hipStreamSynchronize(s.stream);
+                retire_slot(S, s, rc);
+            }
+            progress = true;
+        }
+        if (progress) continue;
+        if (S->inflight == 0)
+            S->wake.wait(lk, [S] { return S->stop || S->pending_requests > 0; });
+        else {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            lk.lock();
+        }
+    }
+}
+
+// Wait until no batch is in flight and nothing is queued (lock held via lk).
+void drain_locked(tmvp_server *S, std::unique_lock<std::mutex> &lk)
+{
+    S->completed.wait(lk, [S] { return S->inflight == 0 && S->pending_requests == 0; });
+}
+
+int grow_store(Store &st, const KemParams &P, int64_t need, bool with_sk)
+{
+    if (need <= st.cap) return TMVP_OK;
+    int64_t cap = st.cap ? st.cap : 1024;
+    while (cap < need) cap *= 2;
+    const size_t abytes = (size_t)P.l * P.l * kN * 2;
+    uint8_t *pk = nullptr, *sk = nullptr, *hpk = nullptr;
+    uint16_t *A = nullptr;
+    if (hipMalloc(&pk, (size_t)cap * P.pk) != hipSuccess || hipMalloc(&hpk, (size_t)cap * 32) != hipSuccess ||
+        hipMalloc(&A, (size_t)cap * abytes) != hipSuccess || (with_sk && hipMalloc(&sk, (size_t)cap * P.sk) != hipSuccess)) {
+        (void)hipFree(pk);
+        (void)hipFree(hpk);
+        (void)hipFree(A);
+        (void)hipFree(sk);
+        return TMVP_ENOMEM;
+    }
+    if (st.n) {
+        (void)hipMemcpy(pk, st.pk, (size_t)st.n * P.pk, hipMemcpyDeviceToDevice);
+        (void)hipMemcpy(hpk, st.hpk, (size_t)st.n * 32, hipMemcpyDeviceToDevice);
+        (void)hipMemcpy(A, st.A, (size_t)st.n * abytes, hipMemcpyDeviceToDevice);
+        if (with_sk) (void)hipMemcpy(sk, st.sk, (size_t)st.n * P.sk, hipMemcpyDeviceToDevice);
+    }
+    (void)hipFree(st.pk);
+    (void)hipFree(st.hpk);
+    (void)hipFree(st.A);
+    (void)hipFree(st.sk);
+    st.pk = pk;
+    st.sk = sk;
+    st.hpk = hpk;
+    st.A = A;
+    st.cap = cap;
+    return TMVP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int tmvp_srv_create(tmvp_ctx *ctx, const tmvp_srv_config *cfg, tmvp_server **out)
+{
+    if (!ctx || !out) return TMVP_EARG;
+    *out = nullptr;
+    tmvp_srv_config c = cfg ? *cfg : tmvp_srv_config{};
+    if (c.hbm_budget < 0 || c.max_batch < 0 || c.slots < 0 || !kem_backend_ok(c.backend)) return TMVP_EARG;
+    if (c.hbm_budget == 0) c.hbm_budget = (int64_t)4 << 30;
+    if (c.max_batch == 0) c.max_batch = 16384;
+    if (c.slots == 0) c.slots = 4;
+    if (c.max_batch > ((int64_t)1 << 24)) return TMVP_EARG;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return TMVP_EHIP;
+    tmvp_server *S = new (std::nothrow) tmvp_server;
+    if (!S) return TMVP_ENOMEM;
+    S->ctx = ctx;
+    S->device = dev;
+    S->backend = c.backend;
+    S->max_batch = c.max_batch;
+    S->t0 = Clock::now();
+    max_sizes(c.max_batch, S->ws_bytes, S->in_bytes, S->out_bytes);
+    const int64_t per_slot = (int64_t)(S->ws_bytes + S->in_bytes + S->out_bytes);
+    int nslots = (int)(c.hbm_budget / per_slot);
+    if (nslots > c.slots) nslots = c.slots;
+    if (nslots < 1) {
+        delete S;
+        return TMVP_ENOMEM; // the budget does not cover one batch of max_batch instances
+    }
+    S->slots.resize(nslots);
+    for (Slot &s : S->slots) {
+        const int rc = alloc_slot(S, s);
+        if (rc) {
+            for (Slot &f : S->slots) free_slot(f);
+            delete S;
+            return rc;
+        }
+    }
+    S->stats.slots = nslots;
+    S->stats.slot_bytes = per_slot;
+    try {
+        S->worker = std::thread(worker_main, S);
+    } catch (...) {
+        for (Slot &f : S->slots) free_slot(f);
+        delete S;
+        return TMVP_ENOMEM;
+    }
+    *out = S;
+    return TMVP_OK;
+}
+
+int tmvp_srv_add_keys(tmvp_server *S, int ps, const uint8_t *h_pk, const uint8_t *h_sk, int64_t n, int64_t *first_id)
+{
+    if (!S || ps < 0 || ps >= TMVP_NUM_PARAM_SETS || n < 0 || (n > 0 && !h_pk)) return TMVP_EARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    drain_locked(S, lk); // batches in flight read the store
+    Store &st = S->store[ps];
+    if (st.n > 0 && st.has_sk != (h_sk != nullptr)) return TMVP_EARG; // all keys of a set with or without sk
+    if (st.n + n > ((int64_t)1 << 31) - 1) return TMVP_EARG;
+    if (first_id) *first_id = st.n;
+    if (n == 0) return TMVP_OK;
+    const KemParams P = kem_params_of(ps);
+    if (st.n == 0) st.has_sk = h_sk != nullptr;
+    int rc = grow_store(st, P, st.n + n, st.has_sk);
+    if (rc) return rc;
+    uint8_t *araw = nullptr;
+    if (hipMalloc(&araw, (size_t)n * P.araw) != hipSuccess) return TMVP_ENOMEM;
+    const size_t abytes = (size_t)P.l * P.l * kN * 2;
+    rc = TMVP_OK;
+    if (hipMemcpy(st.pk + (size_t)st.n * P.pk, h_pk, (size_t)n * P.pk, hipMemcpyHostToDevice) != hipSuccess ||
+        (h_sk && hipMemcpy(st.sk + (size_t)st.n * P.sk, h_sk, (size_t)n * P.sk, hipMemcpyHostToDevice) != hipSuccess))
+        rc = TMVP_EHIP;
+    if (!rc && kem_store_derive(P, st.pk + (size_t)st.n * P.pk, n, st.A + (size_t)st.n * (abytes / 2),
+                                st.hpk + (size_t)st.n * 32, araw, nullptr) != hipSuccess)
+        rc = TMVP_EHIP;
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = TMVP_EHIP;
+    (void)hipFree(araw);
+    if (!rc) st.n += n;
+    return rc;
+}
+
+int tmvp_srv_submit(tmvp_server *S, const tmvp_srv_request *r, uint64_t *id)
+{
+    if (!S || !r) return TMVP_EARG;
+    if (r->op != TMVP_SRV_ENCAPS && r->op != TMVP_SRV_DECAPS) return TMVP_EARG;
+    if (r->param_set < 0 || r->param_set >= TMVP_NUM_PARAM_SETS) return TMVP_EARG;
+    if (r->count < 1 || !r->keys || !r->in || !r->ss || (r->op == TMVP_SRV_ENCAPS && !r->ct)) return TMVP_EARG;
+    std::lock_guard<std::mutex> g(S->mu);
+    if (r->count > S->max_batch) return TMVP_EARG;
+    const Store &st = S->store[r->param_set];
+    if (r->op == TMVP_SRV_DECAPS && !st.has_sk) return TMVP_EARG;
+    for (int64_t i = 0; i < r->count; i++)
+        if (r->keys[i] < 0 || r->keys[i] >= st.n) return TMVP_EARG;
+    Pending p;
+    p.id = S->next_id++;
+    p.req = *r;
+    p.t_submit = S->now_us();
+    S->queue[r->param_set][r->op].push_back(p);
+    S->pending_requests += 1;
+    S->stats.requests_submitted += 1;
+    if (id) *id = p.id;
+    S->wake.notify_one();
+    return TMVP_OK;
+}
+
+int tmvp_srv_poll(tmvp_server *S, tmvp_srv_completion *out, int max, int timeout_us)
+{
+    if (!S || max < 0 || (max > 0 && !out)) return TMVP_EARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (S->done.empty() && timeout_us > 0)
+        S->completed.wait_for(lk, std::chrono::microseconds(timeout_us), [S] { return !S->done.empty(); });
+    int k = 0;
+    while (k < max && !S->done.empty()) {
+        out[k++] = S->done.front();
+        S->done.pop_front();
+    }
+    return k;
+}
+
+int tmvp_srv_drain(tmvp_server *S)
+{
+    if (!S) return TMVP_EARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    drain_locked(S, lk);
+    return TMVP_OK;
+}
+
+int tmvp_srv_get_stats(tmvp_server *S, tmvp_srv_stats *out)
+{
+    if (!S || !out) return TMVP_EARG;
+    std::lock_guard<std::mutex> g(S->mu);
+    *out = S->stats;
+    return TMVP_OK;
+}
+
+int tmvp_srv_destroy(tmvp_server *S)
+{
+    if (!S) return TMVP_EARG;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        drain_locked(S, lk);
+        S->stop = true;
+        S->wake.notify_all();
+    }
+    S->worker.join();
+    for (Slot &s : S->slots) free_slot(s);
+    for (Store &st : S->store) {
+        (void)hipFree(st.pk);
+        (void)hipFree(st.sk);
+        (void)hipFree(st.hpk);
+        (void)hipFree(st.A);
+    }
+    delete S;
+    return TMVP_OK;
+}
+
+} // extern "C"

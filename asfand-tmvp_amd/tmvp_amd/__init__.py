@@ -68,7 +68,34 @@ EXPORTS = ("tmvp_init", "tmvp_destroy", "tmvp_ctx_status", "tmvp_strerror", "tmv
            "tmvp_host_configure", "tmvp_host_set_randomness", "tmvp_host_status", "tmvp_host_release",
            "saber_enc", "saber_dec", "SABLE_enc", "SABLE_dec",
            "tmvp_batch_launch", "tmvp_batch_autotune", "tmvp_graph_begin", "tmvp_graph_end", "tmvp_graph_launch",
-           "tmvp_graph_destroy", "tmvp_kem_reserve")
+           "tmvp_graph_destroy", "tmvp_kem_reserve",
+           "tmvp_srv_create", "tmvp_srv_add_keys", "tmvp_srv_submit", "tmvp_srv_poll", "tmvp_srv_drain",
+           "tmvp_srv_get_stats", "tmvp_srv_destroy")
+
+SRV_ENCAPS, SRV_DECAPS = 0, 1
+
+
+class SrvConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_int64), ("max_batch", ctypes.c_int64), ("slots", ctypes.c_int),
+                ("backend", ctypes.c_int)]
+
+
+class SrvRequest(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int), ("param_set", ctypes.c_int), ("count", ctypes.c_int64),
+                ("keys", ctypes.c_void_p), ("in_", ctypes.c_void_p), ("ct", ctypes.c_void_p),
+                ("ss", ctypes.c_void_p), ("fail", ctypes.c_void_p), ("tag", ctypes.c_uint64)]
+
+
+class SrvCompletion(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("tag", ctypes.c_uint64), ("status", ctypes.c_int),
+                ("batch_requests", ctypes.c_int), ("batch_instances", ctypes.c_int64),
+                ("submit_us", ctypes.c_double), ("start_us", ctypes.c_double), ("done_us", ctypes.c_double)]
+
+
+class SrvStats(ctypes.Structure):
+    _fields_ = [("requests_submitted", ctypes.c_int64), ("requests_done", ctypes.c_int64),
+                ("instances_done", ctypes.c_int64), ("batches", ctypes.c_int64), ("slots", ctypes.c_int64),
+                ("slot_bytes", ctypes.c_int64), ("max_concurrent", ctypes.c_int64)]
 
 OP_POLYMUL, OP_MATVEC, OP_MATVEC_T, OP_INNERPROD = 0, 1, 2, 3
 MAX_WORK = 16
@@ -118,6 +145,13 @@ def load_library(path=LIB_PATH):
         "tmvp_graph_launch": (i, [P, P]),
         "tmvp_graph_destroy": (i, [P]),
         "tmvp_kem_reserve": (i, [P, i, i64, P]),
+        "tmvp_srv_create": (i, [P, ctypes.POINTER(SrvConfig), ctypes.POINTER(P)]),
+        "tmvp_srv_add_keys": (i, [P, i, P, P, i64, ctypes.POINTER(i64)]),
+        "tmvp_srv_submit": (i, [P, ctypes.POINTER(SrvRequest), ctypes.POINTER(ctypes.c_uint64)]),
+        "tmvp_srv_poll": (i, [P, P, i, i]),
+        "tmvp_srv_drain": (i, [P]),
+        "tmvp_srv_get_stats": (i, [P, ctypes.POINTER(SrvStats)]),
+        "tmvp_srv_destroy": (i, [P]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -497,3 +531,88 @@ def run_kem_decaps(ctx, ct, sk, param_set=SABER, backend=MFMA):
     device_synchronize()
     ctx.check()
     return dss.download(np.uint8, (batch, -1)), dfail.download(np.uint8)
+
+
+class Server:
+    """Serving loop over device-resident keys (tmvp_server).  Host arrays of a
+    submitted request are kept alive here until its completion is polled."""
+
+    def __init__(self, ctx, hbm_budget=0, max_batch=0, slots=0, backend=MFMA):
+        self.lib = load_library()
+        self.handle = ctypes.c_void_p()
+        cfg = SrvConfig(hbm_budget, max_batch, slots, backend)
+        rc = self.lib.tmvp_srv_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(self.handle))
+        if rc:
+            raise TmvpError(rc, "tmvp_srv_create")
+        self.live = {}
+
+    def add_keys(self, param_set, pk, sk=None):
+        ps = PARAM_SET_NAMES.get(param_set, param_set)
+        pk = np.ascontiguousarray(pk, np.uint8)
+        n = pk.shape[0]
+        first = ctypes.c_int64()
+        skp = None
+        if sk is not None:
+            sk = np.ascontiguousarray(sk, np.uint8)
+            skp = sk.ctypes.data_as(ctypes.c_void_p)
+        rc = self.lib.tmvp_srv_add_keys(self.handle, int(ps), pk.ctypes.data_as(ctypes.c_void_p), skp, n,
+                                        ctypes.byref(first))
+        if rc:
+            raise TmvpError(rc, "tmvp_srv_add_keys")
+        return first.value
+
+    def submit(self, op, param_set, keys, data, tag=0):
+        """Queue a request; returns (id, outputs) where outputs are the numpy
+        arrays the results land in: (ct, ss) for encaps, (ss, fail) for decaps."""
+        ps = PARAM_SET_NAMES.get(param_set, param_set)
+        p = get_params(ps)
+        keys = np.ascontiguousarray(keys, np.int32)
+        data = np.ascontiguousarray(data, np.uint8)
+        n = keys.shape[0]
+        ss = np.zeros((n, 32), np.uint8)
+        if op == SRV_ENCAPS:
+            a, b = np.zeros((n, p["ct_bytes"]), np.uint8), ss
+            ctp, failp = a.ctypes.data_as(ctypes.c_void_p), None
+            outs = (a, ss)
+        else:
+            fail = np.zeros(n, np.uint8)
+            ctp, failp = None, fail.ctypes.data_as(ctypes.c_void_p)
+            outs = (ss, fail)
+        req = SrvRequest(op, int(ps), n, keys.ctypes.data_as(ctypes.c_void_p), data.ctypes.data_as(ctypes.c_void_p),
+                         ctp, ss.ctypes.data_as(ctypes.c_void_p), failp, tag)
+        rid = ctypes.c_uint64()
+        rc = self.lib.tmvp_srv_submit(self.handle, ctypes.byref(req), ctypes.byref(rid))
+        if rc:
+            raise TmvpError(rc, "tmvp_srv_submit")
+        self.live[rid.value] = (keys, data, outs)
+        return rid.value, outs
+
+    def poll(self, max_n=256, timeout_us=0):
+        buf = (SrvCompletion * max_n)()
+        k = self.lib.tmvp_srv_poll(self.handle, ctypes.cast(buf, ctypes.c_void_p), max_n, int(timeout_us))
+        if k < 0:
+            raise TmvpError(k, "tmvp_srv_poll")
+        out = []
+        for c in buf[:k]:
+            self.live.pop(c.id, None)
+            out.append({"id": c.id, "tag": c.tag, "status": c.status, "batch_requests": c.batch_requests,
+                        "batch_instances": c.batch_instances, "submit_us": c.submit_us, "start_us": c.start_us,
+                        "done_us": c.done_us})
+        return out
+
+    def drain(self):
+        rc = self.lib.tmvp_srv_drain(self.handle)
+        if rc:
+            raise TmvpError(rc, "tmvp_srv_drain")
+
+    def stats(self):
+        st = SrvStats()
+        rc = self.lib.tmvp_srv_get_stats(self.handle, ctypes.byref(st))
+        if rc:
+            raise TmvpError(rc, "tmvp_srv_get_stats")
+        return {f: getattr(st, f) for f, _ in SrvStats._fields_}
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            self.lib.tmvp_srv_destroy(self.handle)
+        self.handle = None

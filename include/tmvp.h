@@ -223,6 +223,67 @@ int tmvp_kem_decaps(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_ss, co
  * (synchronises the device), so later KEM calls on it allocate nothing. */
 int tmvp_kem_reserve(tmvp_ctx *ctx, int param_set, int64_t batch, tmvp_stream_t stream);
 
+/* ---- serving: continuous batching of KEM requests over resident keys ----
+ * Not in the reference (its host functions run one fixed batch per call).
+ * Keys live in HBM per parameter set, with the matrix A and H(pk) derived
+ * once per key.  Requests (one operation, one parameter set, 1..max_batch
+ * instances, each under its own key id) are queued per (set, operation); a
+ * scheduler thread forms each batch from the queue whose oldest request has
+ * waited longest, in arrival order up to max_batch instances, and runs it on
+ * a free slot (a stream with its own workspace and staging).  The HBM budget
+ * sets the number of slots, so admission waits for budget; batches on
+ * different slots overlap on the device.  Host buffers of a request must stay
+ * valid until its completion is polled.  Results are those of
+ * tmvp_kem_encaps / tmvp_kem_decaps for the same keys and inputs, whatever
+ * the batch a request lands in. */
+typedef struct tmvp_server tmvp_server;
+
+typedef struct {
+    int64_t hbm_budget; /* bytes for in-flight batches; 0 = 4 GiB */
+    int64_t max_batch;  /* instances per batch; 0 = 16384 */
+    int slots;          /* concurrent batches at most; 0 = 4 */
+    int backend;        /* tmvp_backend */
+} tmvp_srv_config;
+
+#define TMVP_SRV_ENCAPS 0
+#define TMVP_SRV_DECAPS 1
+
+typedef struct {
+    int op;              /* TMVP_SRV_ENCAPS / TMVP_SRV_DECAPS */
+    int param_set;
+    int64_t count;       /* instances */
+    const int32_t *keys; /* [count] key ids of this parameter set */
+    const uint8_t *in;   /* randomness [count][32] (encaps) / ciphertexts [count][ct_bytes] (decaps) */
+    uint8_t *ct;         /* encaps output [count][ct_bytes] */
+    uint8_t *ss;         /* output [count][32] */
+    uint8_t *fail;       /* decaps output [count] (1 = rejected), may be NULL */
+    uint64_t tag;        /* returned in the completion */
+} tmvp_srv_request;
+
+typedef struct {
+    uint64_t id, tag;
+    int status;              /* TMVP_OK or an error code */
+    int batch_requests;      /* requests in the batch that carried it */
+    int64_t batch_instances; /* instances in that batch */
+    double submit_us, start_us, done_us; /* server clock */
+} tmvp_srv_completion;
+
+typedef struct {
+    int64_t requests_submitted, requests_done, instances_done, batches;
+    int64_t slots, slot_bytes, max_concurrent;
+} tmvp_srv_stats;
+
+int tmvp_srv_create(tmvp_ctx *ctx, const tmvp_srv_config *config, tmvp_server **server);
+/* Appends n keys (sk may be NULL: encapsulation only); ids first_id.. */
+int tmvp_srv_add_keys(tmvp_server *server, int param_set, const uint8_t *h_pk, const uint8_t *h_sk, int64_t n,
+                      int64_t *first_id);
+int tmvp_srv_submit(tmvp_server *server, const tmvp_srv_request *request, uint64_t *id);
+/* Up to max completions, waiting up to timeout_us for the first; returns the count. */
+int tmvp_srv_poll(tmvp_server *server, tmvp_srv_completion *out, int max, int timeout_us);
+int tmvp_srv_drain(tmvp_server *server);
+int tmvp_srv_get_stats(tmvp_server *server, tmvp_srv_stats *out);
+int tmvp_srv_destroy(tmvp_server *server);
+
 /* ---- the reference's host entry points (same names and signatures) ----
  * saber_enc / saber_dec (S/include/cuda_kernel.cuh:6-7) and SABLE_enc /
  * SABLE_dec (SB/include/SABLE_indcpa.cuh:8-10) over the batched KEM above.

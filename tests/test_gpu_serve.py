@@ -1,0 +1,157 @@
+"""GPU checks of the serving loop (tmvp_server): requests of mixed parameter
+sets and operations, submitted together so that they share batches and
+overlap on the device, give byte for byte what each gives alone (through the
+server with nothing else queued, and through the plain batched KEM calls),
+and a sample agrees with the CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+import tmvp_amd as T
+
+pytestmark = pytest.mark.gpu
+
+SETS = ("saber", "sable2")
+NKEYS = 600
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = T.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def keys(ctx):
+    out = {}
+    for i, ps in enumerate(SETS):
+        seeds = np.random.default_rng(40 + i).integers(0, 256, (NKEYS, 96), dtype=np.uint8)
+        out[ps] = T.run_kem_keypair(ctx, seeds, ps, T.MFMA)
+    return out
+
+
+def make_requests(ctx, keys, rng, n_req, max_count):
+    """Random encaps / decaps requests; decaps inputs are honest ciphertexts
+    with every fifth instance tampered."""
+    reqs = []
+    for r in range(n_req):
+        ps = SETS[r % 2]
+        op = T.SRV_ENCAPS if (r // 2) % 2 == 0 else T.SRV_DECAPS
+        n = int(rng.integers(1, max_count + 1))
+        ids = rng.integers(0, NKEYS, n).astype(np.int32)
+        pk, sk = keys[ps][0][ids], keys[ps][1][ids]
+        if op == T.SRV_ENCAPS:
+            data = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+            want = T.run_kem_encaps(ctx, pk, data, ps, T.MFMA)
+        else:
+            ct, _ = T.run_kem_encaps(ctx, pk, rng.integers(0, 256, (n, 32), dtype=np.uint8), ps, T.MFMA)
+            for i in range(0, n, 5):
+                ct[i, int(rng.integers(0, ct.shape[1]))] ^= 0x10
+            data = ct
+            want = T.run_kem_decaps(ctx, ct, sk, ps, T.MFMA)
+        reqs.append((op, ps, ids, data, want))
+    return reqs
+
+
+def run_all(srv, reqs):
+    outs, done = {}, {}
+    for k, (op, ps, ids, data, _) in enumerate(reqs):
+        rid, o = srv.submit(op, ps, ids, data, tag=k)
+        outs[k] = o
+    while len(done) < len(reqs):
+        for c in srv.poll(64, timeout_us=200000):
+            done[c["tag"]] = c
+    return outs, done
+
+
+def test_serving_mix_equals_alone(ctx, keys):
+    rng = np.random.default_rng(5)
+    reqs = make_requests(ctx, keys, rng, 24, 300)
+    srv = T.Server(ctx, max_batch=2048, slots=3)
+    try:
+        for ps in SETS:
+            assert srv.add_keys(ps, *keys[ps]) == 0
+        outs, done = run_all(srv, reqs)
+        assert all(c["status"] == 0 for c in done.values())
+        assert max(c["batch_requests"] for c in done.values()) > 1  # requests shared batches
+        for k, (op, ps, ids, data, want) in enumerate(reqs):
+            for got, exp in zip(outs[k], want):
+                assert np.array_equal(got, exp), (k, op, ps)
+        # the same requests alone
+        for k in range(0, len(reqs), 5):
+            op, ps, ids, data, want = reqs[k]
+            _, o = srv.submit(op, ps, ids, data, tag=1000 + k)
+            srv.drain()
+            c = srv.poll(4, timeout_us=100000)
+            assert len(c) == 1 and c[0]["batch_requests"] == 1 and c[0]["status"] == 0
+            for got, exp in zip(o, outs[k]):
+                assert np.array_equal(got, exp), k
+        st = srv.stats()
+        assert st["requests_done"] == len(reqs) + len(range(0, len(reqs), 5))
+    finally:
+        srv.close()
+    # oracle sample: first encapsulation and first decapsulation request
+    for op in (T.SRV_ENCAPS, T.SRV_DECAPS):
+        k = next(i for i, r in enumerate(reqs) if r[0] == op)
+        _, ps, ids, data, _ = reqs[k]
+        if op == T.SRV_ENCAPS:
+            ct, ss = O.kem_enc_batch(ps, data, keys[ps][0][ids], nthreads=8)
+            assert np.array_equal(outs[k][0], ct) and np.array_equal(outs[k][1], ss)
+        else:
+            ss = O.kem_dec_batch(ps, data, keys[ps][1][ids], nthreads=8)
+            assert np.array_equal(outs[k][0], ss)
+            want_fail = np.zeros(len(ids), np.uint8)
+            want_fail[::5] = 1
+            assert np.array_equal(outs[k][1], want_fail)
+
+
+def test_serving_batches_overlap(ctx, keys):
+    """Large requests of both sets keep several slots busy at once."""
+    rng = np.random.default_rng(6)
+    srv = T.Server(ctx, max_batch=4096, slots=4)
+    try:
+        for ps in SETS:
+            srv.add_keys(ps, *keys[ps])
+        reqs = []
+        for r in range(16):
+            ps = SETS[r % 2]
+            ids = rng.integers(0, NKEYS, 4096).astype(np.int32)
+            reqs.append((T.SRV_ENCAPS, ps, ids, rng.integers(0, 256, (4096, 32), dtype=np.uint8), None))
+        outs, done = run_all(srv, reqs)
+        assert all(c["status"] == 0 for c in done.values())
+        st = srv.stats()
+        assert st["max_concurrent"] >= 2
+        k = 3
+        ct, ss = T.run_kem_encaps(ctx, keys[reqs[k][1]][0][reqs[k][2]], reqs[k][3], reqs[k][1], T.MFMA)
+        assert np.array_equal(outs[k][0], ct) and np.array_equal(outs[k][1], ss)
+    finally:
+        srv.close()
+
+
+def test_serving_errors(ctx, keys):
+    with pytest.raises(T.TmvpError) as e:
+        T.Server(ctx, hbm_budget=1 << 20, max_batch=16384)
+    assert e.value.code == -3
+    srv = T.Server(ctx, max_batch=64, slots=1)
+    try:
+        srv.add_keys("saber", *keys["saber"])
+        srv.add_keys("sable2", keys["sable2"][0])  # encapsulation-only keys
+        rnd = np.zeros((2, 32), np.uint8)
+        for bad in ([0, NKEYS], [-1, 0]):
+            with pytest.raises(T.TmvpError):
+                srv.submit(T.SRV_ENCAPS, "saber", np.array(bad), rnd)
+        with pytest.raises(T.TmvpError):
+            srv.submit(T.SRV_ENCAPS, "saber", np.zeros(65, np.int32), np.zeros((65, 32), np.uint8))
+        with pytest.raises(T.TmvpError):
+            srv.submit(T.SRV_DECAPS, "sable2", np.zeros(1, np.int32), np.zeros((1, 1024), np.uint8))
+        with pytest.raises(T.TmvpError):
+            srv.submit(T.SRV_ENCAPS, "firesaber", np.zeros(1, np.int32), rnd[:1])  # no keys of that set
+        rid, (ct, ss) = srv.submit(T.SRV_ENCAPS, "sable2", np.array([3, 4], np.int32), rnd)
+        srv.drain()
+        c = srv.poll(4, timeout_us=100000)
+        assert c[0]["id"] == rid and c[0]["status"] == 0
+        want_ct, want_ss = O.kem_enc_batch("sable2", rnd, keys["sable2"][0][[3, 4]])
+        assert np.array_equal(ct, want_ct) and np.array_equal(ss, want_ss)
+    finally:
+        srv.close()
