@@ -54,7 +54,18 @@ struct Pending {
     uint64_t id;
     tmvp_srv_request req;
     double t_submit;
+    bool pinned_in, pinned_out; // host buffers the DMA engines can reach directly
 };
+
+// Whether p lies in page-locked host memory from hipHostMalloc.
+bool is_pinned(const void *p)
+{
+    if (!p) return true;
+    hipPointerAttribute_t a;
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    (void)hipGetLastError(); // pageable memory reports an error: clear it
+    return e == hipSuccess && a.type == hipMemoryTypeHost;
+}
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -151,18 +162,35 @@ int launch_slot(tmvp_server *S, Slot &s)
 {
     const KemParams P = kem_params_of(s.ps);
     const int64_t B = s.instances;
-    // pinned staging: keys, then inputs
+    // keys through the pinned staging; inputs by DMA from the request's own
+    // buffer when it is pinned, else through the staging
     int32_t *hkeys = reinterpret_cast<int32_t *>(s.h_in);
     uint8_t *hin = s.h_in + up((size_t)B * 4);
+    uint8_t *din_base = s.d_in + up((size_t)B * 4);
     const size_t in_each = s.op == TMVP_SRV_ENCAPS ? 32 : (size_t)P.ct;
-    int64_t off = 0;
+    int64_t off = 0, staged_lo = -1, staged_hi = -1;
     for (const Pending &p : s.reqs) {
         memcpy(hkeys + off, p.req.keys, (size_t)p.req.count * 4);
-        memcpy(hin + off * in_each, p.req.in, (size_t)p.req.count * in_each);
         off += p.req.count;
     }
-    const size_t in_total = up((size_t)B * 4) + (size_t)B * in_each;
-    if (hipMemcpyAsync(s.d_in, s.h_in, in_total, hipMemcpyHostToDevice, s.stream) != hipSuccess) return TMVP_EHIP;
+    if (hipMemcpyAsync(s.d_in, s.h_in, (size_t)B * 4, hipMemcpyHostToDevice, s.stream) != hipSuccess) return TMVP_EHIP;
+    off = 0;
+    for (const Pending &p : s.reqs) {
+        const size_t bytes = (size_t)p.req.count * in_each;
+        if (p.pinned_in) {
+            if (hipMemcpyAsync(din_base + off * in_each, p.req.in, bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+                return TMVP_EHIP;
+        } else {
+            memcpy(hin + off * in_each, p.req.in, bytes);
+            if (staged_lo < 0) staged_lo = off;
+            staged_hi = off + p.req.count;
+        }
+        off += p.req.count;
+    }
+    if (staged_lo >= 0 &&
+        hipMemcpyAsync(din_base + staged_lo * in_each, hin + staged_lo * in_each,
+                       (size_t)(staged_hi - staged_lo) * in_each, hipMemcpyHostToDevice, s.stream) != hipSuccess)
+        return TMVP_EHIP;
     KemWorkspace W;
     kem_workspace_carve(P, B, s.ws, W);
     const Store &st = S->store[s.ps];
@@ -180,15 +208,37 @@ int launch_slot(tmvp_server *S, Slot &s)
         e = kem_decaps_keyed(P, W, backend, K, dkeys, din, s.d_out, s.d_out + (size_t)B * 32, B, s.d_status,
                              s.stream);
     if (e != hipSuccess) return TMVP_EHIP;
-    if (hipMemcpyAsync(s.h_out, s.d_out, out_total, hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
-        hipMemcpyAsync(s.h_status, s.d_status, sizeof(int), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
+    // results: straight into pinned request buffers, else into the staging
+    bool any_staged = false;
+    const bool enc = s.op == TMVP_SRV_ENCAPS;
+    const uint8_t *dss = enc ? s.d_out + (size_t)B * P.ct : s.d_out, *dfail = s.d_out + (size_t)B * 32;
+    off = 0;
+    for (const Pending &p : s.reqs) {
+        const size_t n = (size_t)p.req.count;
+        if (!p.pinned_out) {
+            any_staged = true;
+        } else {
+            if (enc && hipMemcpyAsync(p.req.ct, s.d_out + off * P.ct, n * P.ct, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+                return TMVP_EHIP;
+            if (hipMemcpyAsync(p.req.ss, dss + off * 32, n * 32, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+                return TMVP_EHIP;
+            if (!enc && p.req.fail &&
+                hipMemcpyAsync(p.req.fail, dfail + off, n, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+                return TMVP_EHIP;
+        }
+        off += (int64_t)n;
+    }
+    if (any_staged && hipMemcpyAsync(s.h_out, s.d_out, out_total, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+        return TMVP_EHIP;
+    if (hipMemcpyAsync(s.h_status, s.d_status, sizeof(int), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
         hipEventRecord(s.done, s.stream) != hipSuccess)
         return TMVP_EHIP;
     return TMVP_OK;
 }
 
-// Copy a finished slot's results to the requests' buffers (lock held).
-void retire_slot(tmvp_server *S, Slot &s, int rc)
+// Copy a finished slot's results to the requests' buffers and describe the
+// completions (worker thread, without the lock: the slot is the worker's).
+void copy_out(tmvp_server *S, Slot &s, int rc, std::vector<tmvp_srv_completion> &out)
 {
     const KemParams P = kem_params_of(s.ps);
     const int64_t B = s.instances;
@@ -199,14 +249,13 @@ void retire_slot(tmvp_server *S, Slot &s, int rc)
     }
     const bool enc = s.op == TMVP_SRV_ENCAPS;
     const uint8_t *hct = s.h_out, *hss = enc ? s.h_out + (size_t)B * P.ct : s.h_out, *hfail = s.h_out + (size_t)B * 32;
-    const double t = S->now_us();
     int64_t off = 0;
     for (const Pending &p : s.reqs) {
         const int64_t n = p.req.count;
-        if (!rc) {
-            if (s.op == TMVP_SRV_ENCAPS) memcpy(p.req.ct, hct + off * P.ct, (size_t)n * P.ct);
+        if (!rc && !p.pinned_out) {
+            if (enc) memcpy(p.req.ct, hct + off * P.ct, (size_t)n * P.ct);
             memcpy(p.req.ss, hss + off * 32, (size_t)n * 32);
-            if (s.op == TMVP_SRV_DECAPS && p.req.fail) memcpy(p.req.fail, hfail + off, (size_t)n);
+            if (!enc && p.req.fail) memcpy(p.req.fail, hfail + off, (size_t)n);
         }
         tmvp_srv_completion c;
         c.id = p.id;
@@ -216,17 +265,26 @@ void retire_slot(tmvp_server *S, Slot &s, int rc)
         c.batch_instances = B;
         c.submit_us = p.t_submit;
         c.start_us = s.t_start;
-        c.done_us = t;
-        S->done.push_back(c);
+        c.done_us = 0;
+        out.push_back(c);
         off += n;
     }
+}
+
+// Publish a retired slot's completions (lock held).
+void publish(tmvp_server *S, Slot &s, std::vector<tmvp_srv_completion> &cs)
+{
+    const double t = S->now_us();
+    for (tmvp_srv_completion &c : cs) {
+        c.done_us = t;
+        S->done.push_back(c);
+    }
     S->stats.requests_done += (int64_t)s.reqs.size();
-    S->stats.instances_done += B;
+    S->stats.instances_done += s.instances;
     S->stats.batches += 1;
     S->inflight -= 1;
     s.busy = false;
     s.reqs.clear();
-    S->completed.notify_all();
 }
 
 // Form the next batch: the (set, op) queue whose head waited longest (lock held).
@@ -258,18 +316,26 @@ bool pick_batch(tmvp_server *S, Slot &s)
     return true;
 }
 
+// The scheduler.  Under the lock it only moves requests and completions
+// between queues; staging copies, launches and result copies run unlocked,
+// so submit() and poll() never wait behind a transfer.
 void worker_main(tmvp_server *S)
 {
     (void)hipSetDevice(S->device);
+    std::vector<Slot *> finished, starting;
+    std::vector<int> ferr;
+    std::vector<std::vector<tmvp_srv_completion>> outs(S->slots.size());
     std::unique_lock<std::mutex> lk(S->mu);
     while (!S->stop) {
-        bool progress = false;
+        finished.clear();
+        ferr.clear();
+        starting.clear();
         for (Slot &s : S->slots) {
             if (!s.busy) continue;
             const hipError_t q = hipEventQuery(s.done);
             if (q == hipErrorNotReady) continue;
-            retire_slot(S, s, q == hipSuccess ? TMVP_OK : TMVP_EHIP);
-            progress = true;
+            finished.push_back(&s);
+            ferr.push_back(q == hipSuccess ? TMVP_OK : TMVP_EHIP);
         }
         for (Slot &s : S->slots) {
             if (s.busy || S->pending_requests == 0) continue;
@@ -278,21 +344,38 @@ void worker_main(tmvp_server *S)
             s.t_start = S->now_us();
             S->inflight += 1;
             if (S->inflight > S->stats.max_concurrent) S->stats.max_concurrent = S->inflight;
-            const int rc = launch_slot(S, s);
-            if (rc) {
-                (void)hipStreamSynchronize(s.stream);
-                retire_slot(S, s, rc);
+            starting.push_back(&s);
+        }
+        if (finished.empty() && starting.empty()) {
+            if (S->inflight == 0)
+                S->wake.wait(lk, [S] { return S->stop || S->pending_requests > 0; });
+            else {
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(10));
+                lk.lock();
             }
-            progress = true;
+            continue;
         }
-        if (progress) continue;
-        if (S->inflight == 0)
-            S->wake.wait(lk, [S] { return S->stop || S->pending_requests > 0; });
-        else {
-            lk.unlock();
-            std::this_thread::sleep_for(std::chrono::microseconds(20));
-            lk.lock();
+        lk.unlock();
+        for (size_t i = 0; i < finished.size(); i++) {
+            const size_t k = (size_t)(finished[i] - S->slots.data());
+            outs[k].clear();
+            copy_out(S, *finished[i], ferr[i], outs[k]);
         }
+        std::vector<Slot *> failed;
+        for (Slot *s : starting) {
+            if (launch_slot(S, *s)) {
+                (void)hipStreamSynchronize(s->stream);
+                const size_t k = (size_t)(s - S->slots.data());
+                outs[k].clear();
+                copy_out(S, *s, TMVP_EHIP, outs[k]);
+                failed.push_back(s);
+            }
+        }
+        lk.lock();
+        for (Slot *s : finished) publish(S, *s, outs[(size_t)(s - S->slots.data())]);
+        for (Slot *s : failed) publish(S, *s, outs[(size_t)(s - S->slots.data())]);
+        if (!finished.empty() || !failed.empty()) S->completed.notify_all();
     }
 }
 
@@ -425,6 +508,10 @@ int tmvp_srv_submit(tmvp_server *S, const tmvp_srv_request *r, uint64_t *id)
     if (r->op != TMVP_SRV_ENCAPS && r->op != TMVP_SRV_DECAPS) return TMVP_EARG;
     if (r->param_set < 0 || r->param_set >= TMVP_NUM_PARAM_SETS) return TMVP_EARG;
     if (r->count < 1 || !r->keys || !r->in || !r->ss || (r->op == TMVP_SRV_ENCAPS && !r->ct)) return TMVP_EARG;
+    // outside the lock: pointer queries of the caller's buffers
+    const bool pinned_in = is_pinned(r->in);
+    const bool pinned_out = is_pinned(r->ss) && (r->op != TMVP_SRV_ENCAPS || is_pinned(r->ct)) &&
+                            (r->op != TMVP_SRV_DECAPS || is_pinned(r->fail));
     std::lock_guard<std::mutex> g(S->mu);
     if (r->count > S->max_batch) return TMVP_EARG;
     const Store &st = S->store[r->param_set];
@@ -435,6 +522,8 @@ int tmvp_srv_submit(tmvp_server *S, const tmvp_srv_request *r, uint64_t *id)
     p.id = S->next_id++;
     p.req = *r;
     p.t_submit = S->now_us();
+    p.pinned_in = pinned_in;
+    p.pinned_out = pinned_out;
     S->queue[r->param_set][r->op].push_back(p);
     S->pending_requests += 1;
     S->stats.requests_submitted += 1;

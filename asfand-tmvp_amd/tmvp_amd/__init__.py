@@ -176,6 +176,8 @@ def hip():
             "hipMemsetAsync": (i, [P, i, sz, P]),
             "hipDeviceSynchronize": (i, []),
             "hipStreamCreate": (i, [ctypes.POINTER(P)]),
+            "hipHostMalloc": (i, [ctypes.POINTER(P), sz, ctypes.c_uint]),
+            "hipHostFree": (i, [P]),
             "hipStreamDestroy": (i, [P]),
             "hipStreamSynchronize": (i, [P]),
             "hipEventCreate": (i, [ctypes.POINTER(P)]),
@@ -533,6 +535,30 @@ def run_kem_decaps(ctx, ct, sk, param_set=SABER, backend=MFMA):
     return dss.download(np.uint8, (batch, -1)), dfail.download(np.uint8)
 
 
+class _PinnedBlock:
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        _hipcheck(hip().hipHostMalloc(ctypes.byref(self.ptr), max(1, nbytes), 0), "hipHostMalloc")
+        self.nbytes = nbytes
+
+    def __del__(self):
+        if self.ptr is not None and self.ptr.value:
+            hip().hipHostFree(self.ptr)
+            self.ptr = None
+
+
+def pinned_array(shape, dtype=np.uint8):
+    """A numpy array in page-locked host memory (hipHostMalloc); the memory is
+    freed when the array and every view of it are gone."""
+    dtype = np.dtype(dtype)
+    nbytes = int(np.prod(shape)) * dtype.itemsize
+    blk = _PinnedBlock(nbytes)
+    raw = (ctypes.c_uint8 * max(1, nbytes)).from_address(blk.ptr.value)
+    raw._owner = blk
+    arr = np.frombuffer(raw, np.uint8, count=nbytes).view(dtype).reshape(shape)
+    return arr
+
+
 class Server:
     """Serving loop over device-resident keys (tmvp_server).  Host arrays of a
     submitted request are kept alive here until its completion is polled."""
@@ -561,21 +587,21 @@ class Server:
             raise TmvpError(rc, "tmvp_srv_add_keys")
         return first.value
 
-    def submit(self, op, param_set, keys, data, tag=0):
+    def submit(self, op, param_set, keys, data, tag=0, out=None):
         """Queue a request; returns (id, outputs) where outputs are the numpy
-        arrays the results land in: (ct, ss) for encaps, (ss, fail) for decaps."""
+        arrays the results land in: (ct, ss) for encaps, (ss, fail) for decaps
+        (`out` may supply them, e.g. views of pinned_array memory)."""
         ps = PARAM_SET_NAMES.get(param_set, param_set)
         p = get_params(ps)
         keys = np.ascontiguousarray(keys, np.int32)
         data = np.ascontiguousarray(data, np.uint8)
         n = keys.shape[0]
-        ss = np.zeros((n, 32), np.uint8)
         if op == SRV_ENCAPS:
-            a, b = np.zeros((n, p["ct_bytes"]), np.uint8), ss
-            ctp, failp = a.ctypes.data_as(ctypes.c_void_p), None
-            outs = (a, ss)
+            ct, ss = out if out is not None else (np.zeros((n, p["ct_bytes"]), np.uint8), np.zeros((n, 32), np.uint8))
+            ctp, failp = ct.ctypes.data_as(ctypes.c_void_p), None
+            outs = (ct, ss)
         else:
-            fail = np.zeros(n, np.uint8)
+            ss, fail = out if out is not None else (np.zeros((n, 32), np.uint8), np.zeros(n, np.uint8))
             ctp, failp = None, fail.ctypes.data_as(ctypes.c_void_p)
             outs = (ss, fail)
         req = SrvRequest(op, int(ps), n, keys.ctypes.data_as(ctypes.c_void_p), data.ctypes.data_as(ctypes.c_void_p),

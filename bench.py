@@ -13,6 +13,17 @@ Timing uses HIP events recorded on the stream the kernels are launched on.
 `cpu_baseline` times the CPU oracle (a C restatement of the reference's
 Toom-4/schoolbook path) on a bounded sample of the same workload.
 
+`mixed` reports the mixed-parameter workload: 4096 LightSaber, 4096 Saber and
+4096 FireSaber instances, each a MatrixVectorMul rounded to p plus an
+InnerProd mod p, as one persistent launch (default and autotuned plan, and
+replayed from a hipGraph) beside the six separate launches.
+
+`serving` drives the serving loop with Poisson arrivals at a fixed rate:
+half Saber, half Sable type 2, half encapsulations, half decapsulations,
+request sizes uniform in [1, 4096], keys drawn from 100k device-resident keys
+per set; it reports p50/p99 request latency (submit to results in host
+memory) and instances/s.
+
 `kem` reports the full batched Saber KEM at 16384 instances (key generation,
 encapsulation, decapsulation with all sampling, hashing, codecs and products on
 the GPU; seeds and keys resident in HBM): ops/s per step and p50/p99 of the
@@ -190,6 +201,171 @@ def kem_bench(ctx, stream, batch=16384, reps=40, ps="saber", cpu_seconds=4.0, wi
     return out
 
 
+# Algorithmic bytes per instance of the mixed workload: A + s + b' + b + v.
+MIXED_BYTES = {2: 5632, 3: 9728, 4: 14848}
+
+
+def mixed_bench(ctx, stream, batch=4096, steps=50):
+    rng = np.random.default_rng(6)
+    bufs, descs, singles = [], [], []
+    total_bytes = 0
+    for ps, l, mu in (("lightsaber", 2, 10), ("saber", 3, 8), ("firesaber", 4, 6)):
+        A = rng.integers(0, 8192, (batch, l * l * N), dtype=np.uint16)
+        bits = rng.integers(0, 1 << mu, (batch, l * N))
+        half = mu // 2
+        pc = lambda x: np.unpackbits(x.astype(">u2").view(np.uint8).reshape(*x.shape, 2), axis=-1).sum(-1)
+        s = (pc(bits & ((1 << half) - 1)) - pc(bits >> half)).astype(np.int16).view(np.uint16)
+        b = rng.integers(0, 1024, (batch, l * N), dtype=np.uint16)
+        dA, ds_, db = T.DeviceBuffer.from_numpy(A), T.DeviceBuffer.from_numpy(s), T.DeviceBuffer.from_numpy(b)
+        dbp, dv = T.DeviceBuffer(batch * l * N * 2), T.DeviceBuffer(batch * N * 2)
+        bufs += [dA, ds_, db, dbp, dv]
+        descs.append(T.work(T.OP_MATVEC, dbp, dA, ds_, batch, ps, 1))
+        descs.append(T.work(T.OP_INNERPROD, dv, db, ds_, batch, ps, T.IP_RAW))
+        singles.append((dbp, dA, ds_, db, dv, ps))
+        total_bytes += MIXED_BYTES[l] * batch
+
+    def separate():
+        for dbp, dA, ds_, db, dv, ps in singles:
+            ctx.matvec(dbp, dA, ds_, batch, ps, T.MFMA, round_to_p=True, stream=stream)
+            ctx.innerprod(dv, db, ds_, batch, ps, T.MFMA, T.IP_RAW, stream=stream)
+
+    def fused():
+        ctx.batch_launch(descs, T.MFMA, stream=stream)
+
+    out = {"workload": f"{batch} each LightSaber/Saber/FireSaber: MatVec round-to-p + InnerProd mod p",
+           "instances": 3 * batch, "algorithmic_bytes": total_bytes}
+    ms_sep = time_launches(separate, stream, steps, 5) / steps
+    ms_def = time_launches(fused, stream, steps, 5) / steps
+    fused()
+    stream.synchronize()
+    ref = [x.download(np.uint16) for x in bufs[3::5] + bufs[4::5]]
+    separate()
+    stream.synchronize()
+    same = all(np.array_equal(a, x.download(np.uint16)) for a, x in zip(ref, bufs[3::5] + bufs[4::5]))
+    tuned = ctx.batch_autotune(descs, stream=stream)
+    ms_tuned = time_launches(fused, stream, steps, 5) / steps
+    g = ctx.capture(stream, fused)
+    ms_graph = time_launches(lambda: g.launch(stream), stream, steps, 5) / steps
+    g.destroy()
+    ctx.check(stream)
+    best = min(ms_def, ms_tuned, ms_graph)
+    out.update({
+        "separate_launches_ms": ms_sep, "persistent_default_ms": ms_def, "persistent_autotuned_ms": ms_tuned,
+        "autotune_best_ms": tuned, "graph_replay_ms": ms_graph,
+        "instances_per_s": 3 * batch / (best / 1e3),
+        "achieved_GBs": total_bytes / (best / 1e3) / 1e9,
+        "hbm_frac": total_bytes / (best / 1e3) / 1e9 / HBM_PEAK_GBS,
+        "outputs_equal_separate": bool(same),
+    })
+    for x in bufs:
+        x.free()
+    if not same:
+        print("[bench] ERROR: mixed launch differs from separate launches", file=sys.stderr)
+        sys.exit(1)
+    return out
+
+
+def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, seed=9):
+    sets = ("saber", "sable2")
+    rng = np.random.default_rng(seed)
+    srv = T.Server(ctx, max_batch=16384, slots=4)
+    pools = {}
+    try:
+        t0 = time.perf_counter()
+        for ps in sets:
+            p = T.get_params(ps)
+            pk, sk = T.run_kem_keypair(ctx, rng.integers(0, 256, (nkeys, 96), dtype=np.uint8), ps, T.MFMA)
+            srv.add_keys(ps, pk, sk)
+            npool = 16384
+            ids = rng.integers(0, nkeys, npool).astype(np.int32)
+            rnd = T.pinned_array((npool, 32))
+            rnd[:] = rng.integers(0, 256, (npool, 32), dtype=np.uint8)
+            ct = T.pinned_array((npool, p["ct_bytes"]))
+            ct[:] = T.run_kem_encaps(ctx, pk[ids], rng.integers(0, 256, (npool, 32), dtype=np.uint8), ps, T.MFMA)[0]
+            pools[ps] = (ids, rnd, ct)
+            del pk, sk
+        setup_s = time.perf_counter() - t0
+        n_req = int(qps * seconds)
+        gaps = rng.exponential(1.0 / qps, n_req)
+        arrivals = np.cumsum(gaps)
+        plan = []
+        for k in range(n_req):
+            ps = sets[int(rng.integers(0, 2))]
+            op = int(rng.integers(0, 2))
+            n = int(rng.integers(1, max_count + 1))
+            off = int(rng.integers(0, 16384 - n + 1))
+            ids, rnd, ct = pools[ps]
+            plan.append((op, ps, ids[off:off + n], (rnd if op == T.SRV_ENCAPS else ct)[off:off + n]))
+        # warm-up: one small and one full-size request of every (set, operation)
+        for ps in sets:
+            ids, rnd, ct = pools[ps]
+            for op in (T.SRV_ENCAPS, T.SRV_DECAPS):
+                for n in (1, max_count):
+                    srv.submit(op, ps, ids[:n], (rnd if op == T.SRV_ENCAPS else ct)[:n], tag=1 << 40)
+        srv.drain()
+        while srv.poll(256):
+            pass
+        done = []
+        start = time.perf_counter()
+        # outputs land in a pinned ring (results are not read back here; the
+        # serving tests check them)
+        ring = T.pinned_array((256 << 20,))
+        ring_off = 0
+
+        def outputs(op, ps, n):
+            nonlocal ring_off
+            ctb = T.get_params(ps)["ct_bytes"] if op == T.SRV_ENCAPS else 1
+            need = n * (ctb + 32)
+            if ring_off + need > ring.size:
+                ring_off = 0
+            a = ring[ring_off:ring_off + n * ctb].reshape(n, ctb)
+            b = ring[ring_off + n * ctb:ring_off + need].reshape(n, 32)
+            ring_off += need
+            return (a, b) if op == T.SRV_ENCAPS else (b, a.reshape(n))
+
+        next_poll = 0.0
+        for k, (op, ps, ids, data) in enumerate(plan):
+            while True:
+                now = time.perf_counter() - start
+                if now >= arrivals[k]:
+                    break
+                if now >= next_poll:  # collect completions at most every 0.5 ms
+                    done += srv.poll(256)
+                    next_poll = now + 5e-4
+            srv.submit(op, ps, ids, data, tag=k, out=outputs(op, ps, len(ids)))
+        submit_lag_ms = (time.perf_counter() - start - arrivals[-1]) * 1e3
+        while len(done) < n_req:
+            done += srv.poll(256, timeout_us=100000)
+        st = srv.stats()
+        st["batches"] -= 8
+        st["instances_done"] -= 4 * (1 + max_count)
+    finally:
+        srv.close()
+    bad = sum(1 for c in done if c["status"] != 0)
+    lat = sorted((c["done_us"] - c["submit_us"]) / 1e3 for c in done)
+    queue = sorted((c["start_us"] - c["submit_us"]) / 1e3 for c in done)
+    service = sorted((c["done_us"] - c["start_us"]) / 1e3 for c in done)
+    span_s = (max(c["done_us"] for c in done) - min(c["submit_us"] for c in done)) / 1e6
+    inst = sum(len(x[2]) for x in plan)
+    by_op = {}
+    for name, op in (("encaps", T.SRV_ENCAPS), ("decaps", T.SRV_DECAPS)):
+        l = sorted((c["done_us"] - c["submit_us"]) / 1e3 for c in done if plan[c["tag"]][0] == op)
+        by_op[name] = {"p50_ms": percentile(l, 50), "p99_ms": percentile(l, 99)}
+    return {
+        "workload": f"Poisson {qps:.0f} req/s for {seconds:.0f} s; 50% Saber / 50% Sable-2, 50% encaps / 50% decaps, "
+                    f"request size U[1,{max_count}], {nkeys} resident keys per set",
+        "requests": n_req, "instances": inst, "errors": bad,
+        "offered_requests_per_s": qps, "submit_lag_ms": submit_lag_ms,
+        "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99), "max_ms": lat[-1],
+        "p90_ms": percentile(lat, 90), "p999_ms": percentile(lat, 99.9),
+        "queue_ms": {"p50": percentile(queue, 50), "p99": percentile(queue, 99)},
+        "service_ms": {"p50": percentile(service, 50), "p99": percentile(service, 99)},
+        "instances_per_s": inst / span_s, "requests_per_s": n_req / span_s,
+        "by_op": by_op, "batches": st["batches"], "mean_batch_instances": st["instances_done"] / max(1, st["batches"]),
+        "slots": st["slots"], "max_concurrent_batches": st["max_concurrent"], "setup_s": setup_s,
+    }
+
+
 def calib_copy(mib):
     """A plain copy of known size: read mib MiB, write mib MiB (runtime copy kernel).
     Its FETCH_SIZE / WRITE_SIZE readings give the scale of those counters."""
@@ -212,6 +388,10 @@ def main():
     ap.add_argument("--no-kem", action="store_true", help="skip the batched KEM leg")
     ap.add_argument("--kem-batch", type=int, default=16384)
     ap.add_argument("--kem-reps", type=int, default=40)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-parameter leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-qps", default="2000,6000", help="offered request rates, comma separated")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -281,11 +461,25 @@ def main():
         "mfma_tops_useful": MACS_PER_INSTANCE * 2 * value / 1e12,
         "outputs_equal": same,
     }
+    if not args.no_mixed:
+        print("[bench] mixed-parameter batch ...", file=sys.stderr, flush=True)
+        result["mixed"] = mixed_bench(ctx, stream)
+        print(f"[bench] mixed: separate {result['mixed']['separate_launches_ms'] * 1e3:.1f} us, persistent "
+              f"{result['mixed']['persistent_autotuned_ms'] * 1e3:.1f} us, graph "
+              f"{result['mixed']['graph_replay_ms'] * 1e3:.1f} us", file=sys.stderr, flush=True)
     if not args.no_kem:
         print(f"[bench] KEM B={args.kem_batch} reps={args.kem_reps} ...", file=sys.stderr, flush=True)
         result["kem"] = kem_bench(ctx, stream, args.kem_batch, args.kem_reps, with_cpu=not args.no_cpu)
         print(f"[bench] KEM encaps+decaps p99 {result['kem']['encaps_decaps_ms']['p99']:.2f} ms", file=sys.stderr,
               flush=True)
+    if not args.no_serving:
+        result["serving"] = []
+        for qps in (float(x) for x in str(args.serve_qps).split(",")):
+            print(f"[bench] serving {qps:.0f} req/s ...", file=sys.stderr, flush=True)
+            sv = serving_bench(ctx, qps, args.serve_seconds)
+            result["serving"].append(sv)
+            print(f"[bench] serving p50 {sv['p50_ms']:.2f} ms p99 {sv['p99_ms']:.2f} ms, "
+                  f"{sv['instances_per_s'] / 1e6:.2f} M instances/s", file=sys.stderr, flush=True)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline()
