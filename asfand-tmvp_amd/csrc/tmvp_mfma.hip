@@ -78,14 +78,33 @@ __device__ __forceinline__ uint32_t neg_bytes(uint32_t v)
 }
 __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) { return __builtin_amdgcn_perm(hi, lo, sel); }
 
-// Epilogue on x = exact sum (+ h already added for the rounding epilogues).
-template <int EPI>
-__device__ __forceinline__ uint32_t epi(const Job &J, uint32_t x, uint32_t old, uint32_t aux)
+// Per-halfword helpers on packed uint16 pairs (arithmetic mod 2^16).
+__device__ __forceinline__ uint32_t shl16x2(uint32_t a, uint32_t n)
 {
-    if (EPI == kEpiQ2P) return __builtin_amdgcn_ubfe(x, J.eq - J.ep, J.ep);
-    if (EPI == kEpiEnc) return __builtin_amdgcn_ubfe(x - ((aux & 1u) << (J.ep - 1)), J.ep - J.et, J.et);
-    if (EPI == kEpiDec) return __builtin_amdgcn_ubfe(x - ((aux & ((1u << J.et) - 1u)) << (J.ep - J.et)), J.ep - 1, 1);
-    return (x + old) & ((1u << J.w) - 1u);
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) << __builtin_bit_cast(u16x2, n));
+}
+__device__ __forceinline__ uint32_t sub16x2(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
+}
+
+// Epilogue on two outputs at once: y holds (exact sum + h) mod 2^16 for two
+// rows (h, the rounding constant, enters through the accumulator), `old` the
+// current outputs (accumulate) and `aux` the two aux words.
+template <int EPI>
+__device__ __forceinline__ uint32_t epi2(const Job &J, uint32_t y, uint32_t old, uint32_t aux)
+{
+    const uint32_t two = 0x00010001u;
+    if (EPI == kEpiQ2P) return shr16x2(y, J.eq - J.ep) & (((1u << J.ep) - 1u) * two);
+    if (EPI == kEpiEnc) {
+        const uint32_t m = shl16x2(aux & two, (uint32_t)(J.ep - 1) * two);
+        return shr16x2(sub16x2(y, m), J.ep - J.et) & (((1u << J.et) - 1u) * two);
+    }
+    if (EPI == kEpiDec) {
+        const uint32_t c = shl16x2(aux & (((1u << J.et) - 1u) * two), (uint32_t)(J.ep - J.et) * two);
+        return shr16x2(sub16x2(y, c), J.ep - 1) & two;
+    }
+    return (J.accumulate ? add16x2(y, old) : y) & (((1u << J.w) - 1u) * two);
 }
 
 // Per-wave LDS slice: the E blocks of every row for ONE secret term j, and the
@@ -228,16 +247,26 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     stage(0, 0);
     if (inst + stride < J.batch) fetch_term(inst + stride, 0);
     wave_fence();
+    // Accumulators start at the rounding constant h in the lo-limb lanes, so
+    // the epilogue never adds it.
+    const int hinit = (EPI != kEpiMod && limb == 0) ? J.h : 0;
+    const v4i init = {hinit, hinit, hinit, hinit};
+    // Epilogue roles: lo-limb lanes finish rows v = 0, 1 of their group, hi-limb
+    // lanes rows v = 2, 3; each sends the partner the other pair.
+    const uint32_t lshift = limb ? 0x00070007u : 0u;
     for (;;) {
         const int64_t next = inst + stride;
         v4i acc[NR][2];
 #pragma unroll
         for (int j = 0; j < NT; j++) {
             const uint8_t *const E = slice0 + buf * S::BYTES;
-            const uint8_t *const X = E + S::E_BYTES;
             // A operand: Toeplitz tiles of s_j for the four K-chunks and both halves,
-            // 16 bytes from a dword-aligned window each.
-            const uint32_t *aw = reinterpret_cast<const uint32_t *>(X) + abase;
+            // 16 bytes from a dword-aligned window each.  The window's byte offset is
+            // made opaque so that the 16 reads share one address register and carry
+            // their constant parts as instruction offsets.
+            int aoff = buf * S::BYTES + S::E_BYTES + 4 * abase;
+            asm volatile("" : "+v"(aoff));
+            const uint32_t *aw = reinterpret_cast<const uint32_t *>(slice0 + aoff);
             v4i afr[4][2];
 #pragma unroll
             for (int p = 0; p < 4; p++)
@@ -269,10 +298,9 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
                 }
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
-                    const v4i zero = {0, 0, 0, 0};
                     const bool first = (j == 0 && p == 0);
-                    acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][0], bcur[r], first ? zero : acc[r][0], 0, 0, 0);
-                    acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][1], bcur[r], first ? zero : acc[r][1], 0, 0, 0);
+                    acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][0], bcur[r], first ? init : acc[r][0], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][1], bcur[r], first ? init : acc[r][1], 0, 0, 0);
                 }
                 if (p < 3) {
 #pragma unroll
@@ -295,32 +323,26 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
             wave_fence();
         }
 
-        // ---- epilogue: lanes with limb 0 own block column R; rows i = 16h + 4g + v.
-        const uint32_t hadd = (EPI != kEpiMod) ? (uint32_t)J.h : 0u; // h1 or h2 of the rounding
-        uint16_t *obase = J.out + inst * (NR * kN) + 32 * R + 4 * g;
-        const uint16_t *xbase = (EPI == kEpiEnc || EPI == kEpiDec) ? J.aux + inst * kN + 32 * R + 4 * g : nullptr;
+        // ---- epilogue, in packed 16-bit arithmetic (every output is mod 2^w,
+        //      w <= 14): lane (R, limb, g) holds rows i = 16h + 4g + v of block
+        //      column R for its limb.  Both lanes of a pair scale their own limb
+        //      (hi by 128), keep one row pair and send the other across
+        //      row_ror:8, so each finishes two rows: lo lanes v = 0,1, hi v = 2,3.
+        uint16_t *obase = J.out + inst * (NR * kN) + 32 * R + 4 * g + 2 * limb;
+        const uint16_t *xbase = (EPI == kEpiEnc || EPI == kEpiDec) ? J.aux + inst * kN + 32 * R + 4 * g + 2 * limb : nullptr;
 #pragma unroll
         for (int r = 0; r < NR; r++) {
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                // hi-limb lanes contribute 128 * hi (+ the rounding constant); row_ror:8
-                // brings them next to their lo-limb lanes.
-                uint32_t val[4];
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const uint32_t scaled = ((uint32_t)acc[r][h][v] << 7) + hadd;
-                    val[v] = (uint32_t)acc[r][h][v] + (uint32_t)__builtin_amdgcn_mov_dpp((int)scaled, 0x128, 0xf, 0xf, false);
-                }
-                if (limb == 0) {
-                    uint2 oldv = make_uint2(0u, 0u), auxv = make_uint2(0u, 0u);
-                    if (EPI == kEpiMod && J.accumulate) oldv = *(const uint2 *)(obase + r * kN + 16 * h);
-                    if (EPI == kEpiEnc || EPI == kEpiDec) auxv = *(const uint2 *)(xbase + 16 * h);
-                    const uint32_t o0 = epi<EPI>(J, val[0], oldv.x & 0xffffu, auxv.x & 0xffffu);
-                    const uint32_t o1 = epi<EPI>(J, val[1], oldv.x >> 16, auxv.x >> 16);
-                    const uint32_t o2 = epi<EPI>(J, val[2], oldv.y & 0xffffu, auxv.y & 0xffffu);
-                    const uint32_t o3 = epi<EPI>(J, val[3], oldv.y >> 16, auxv.y >> 16);
-                    *(u32x2 *)(obase + r * kN + 16 * h) = u32x2{o0 | (o1 << 16), o2 | (o3 << 16)};
-                }
+                const uint32_t p01 = shl16x2(perm((uint32_t)acc[r][h][1], (uint32_t)acc[r][h][0], 0x05040100u), lshift);
+                const uint32_t p23 = shl16x2(perm((uint32_t)acc[r][h][3], (uint32_t)acc[r][h][2], 0x05040100u), lshift);
+                const uint32_t keep = limb ? p23 : p01;
+                const uint32_t send = limb ? p01 : p23;
+                const uint32_t y = add16x2(keep, (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0x128, 0xf, 0xf, false));
+                uint32_t old = 0u, auxv = 0u;
+                if (EPI == kEpiMod && J.accumulate) old = *(const uint32_t *)(obase + r * kN + 16 * h);
+                if (EPI == kEpiEnc || EPI == kEpiDec) auxv = *(const uint32_t *)(xbase + 16 * h);
+                *(uint32_t *)(obase + r * kN + 16 * h) = epi2<EPI>(J, y, old, auxv);
             }
         }
         inst = next;
