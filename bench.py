@@ -76,6 +76,23 @@ def synthetic_inputs(batch, seed=1):
     return A, s
 
 
+def measured_peaks():
+    """HBM streaming rate with this workload's traffic shape and the int8
+    matrix-core rate, measured on this device (libtmvp_peaks.so)."""
+    import ctypes
+    path = os.path.join(ROOT, "asfand-tmvp_amd", "libtmvp_peaks.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    us, gbs, tops = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    if lib.tmvp_peak_stream(ctypes.c_int64(BATCH), 20, ctypes.byref(us), ctypes.byref(gbs)):
+        return None
+    if lib.tmvp_peak_mfma_i8(5, ctypes.byref(tops)):
+        return None
+    return {"hbm_stream_GBs": gbs.value, "hbm_stream_us": us.value, "mfma_i8_tops": tops.value,
+            "stream_shape": "per instance 6144 B read + 1536 B written, one wave per instance, B=65536"}
+
+
 def time_launches(fn, stream, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -461,6 +478,12 @@ def main():
         "mfma_tops_useful": MACS_PER_INSTANCE * 2 * value / 1e12,
         "outputs_equal": same,
     }
+    peaks = measured_peaks()
+    if peaks:
+        executed = MACS_PER_INSTANCE * 2 * 2 * value / 1e12  # two int8 limbs per product
+        result["roofline"]["measured"] = {
+            **peaks, "frac_of_stream": achieved / peaks["hbm_stream_GBs"],
+            "mfma_executed_tops": executed, "mfma_util_of_measured": executed / peaks["mfma_i8_tops"]}
     if not args.no_mixed:
         print("[bench] mixed-parameter batch ...", file=sys.stderr, flush=True)
         result["mixed"] = mixed_bench(ctx, stream)
