@@ -103,6 +103,7 @@ KemParams kem_params(int ps)
     P.sk = P.indcpa_sk + P.pk + 64;
     P.ct = P.pvc + p.et * 32;
     P.araw = p.l * p.l * p.eq * 32;
+    P.sraw = p.l * p.mu * 32;
     return P;
 }
 

@@ -1,11 +1,18 @@
-// Keccak-f[1600] and the FIPS 202 sponges (SHAKE-128, SHA3-256, SHA3-512)
-// for one thread per sponge, state in registers.
+// Keccak-f[1600] and the FIPS 202 sponges (SHAKE-128, SHA3-256, SHA3-512),
+// one state split across a pair of adjacent lanes: lane p = lane & 1 holds
+// the low (p = 0) or high (p = 1) 32 bits of each of the 25 words.
+//
+// A 64-bit rotation by r then needs the partner's half, fetched with one DPP
+// swap of adjacent lanes, and one v_alignbit: for 0 < r < 32 the new half is
+// alignbit(own, partner, 32 - r) in both lanes, for r > 32 it is
+// alignbit(partner, own, 64 - r).  A round costs about 160 VALU instructions
+// per lane, against about 290 for a whole state in one lane, and a batch gets
+// twice the waves.
 //
 // Round constants come from the rc(t) LFSR of FIPS 202 Algorithm 5 and the
 // rho offsets from the (x, y) -> (y, 2x + 3y) walk of Algorithm 2, both
-// evaluated at compile time.  All sponge inputs and outputs are 8-byte
-// aligned per instance (every Saber/Sable key, ciphertext and seed size is a
-// multiple of 8), so lanes are moved as whole little-endian 64-bit words.
+// evaluated at compile time.  Both lanes of a pair must execute every call
+// together (the DPP swap reads the partner's register).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -57,41 +64,56 @@ constexpr Tables kT = make_tables();
 static_assert(kT.rc[0] == 1ull, "rc(0)");
 static_assert(kT.rho[1] == 1 && kT.rho[2] == 62, "rho offsets");
 
-__device__ __constant__ uint64_t kRC[24] = {
-    kT.rc[0],  kT.rc[1],  kT.rc[2],  kT.rc[3],  kT.rc[4],  kT.rc[5],  kT.rc[6],  kT.rc[7],
-    kT.rc[8],  kT.rc[9],  kT.rc[10], kT.rc[11], kT.rc[12], kT.rc[13], kT.rc[14], kT.rc[15],
-    kT.rc[16], kT.rc[17], kT.rc[18], kT.rc[19], kT.rc[20], kT.rc[21], kT.rc[22], kT.rc[23]};
+// Round constants as (low, high) halves.
+__device__ __constant__ uint32_t kRC2[48] = {
+#define TMVP_RC(i) (uint32_t)kT.rc[i], (uint32_t)(kT.rc[i] >> 32)
+    TMVP_RC(0),  TMVP_RC(1),  TMVP_RC(2),  TMVP_RC(3),  TMVP_RC(4),  TMVP_RC(5),  TMVP_RC(6),  TMVP_RC(7),
+    TMVP_RC(8),  TMVP_RC(9),  TMVP_RC(10), TMVP_RC(11), TMVP_RC(12), TMVP_RC(13), TMVP_RC(14), TMVP_RC(15),
+    TMVP_RC(16), TMVP_RC(17), TMVP_RC(18), TMVP_RC(19), TMVP_RC(20), TMVP_RC(21), TMVP_RC(22), TMVP_RC(23)
+#undef TMVP_RC
+};
 
-template <int N> __device__ __forceinline__ uint64_t rotl(uint64_t v)
+// The partner lane's value (DPP quad_perm [1, 0, 3, 2]).
+__device__ __forceinline__ uint32_t partner(uint32_t v)
 {
-    if constexpr (N == 0) return v;
-    else return (v << N) | (v >> (64 - N));
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, false);
+}
+
+// This lane's half of rotl64(word, R), given its half and the partner's.
+template <int R> __device__ __forceinline__ uint32_t rotl2(uint32_t own, uint32_t other)
+{
+    if constexpr (R == 0) return own;
+    else if constexpr (R < 32) return __builtin_amdgcn_alignbit(own, other, 32 - R);
+    else if constexpr (R == 32) return other;
+    else return __builtin_amdgcn_alignbit(other, own, 64 - R);
 }
 
 // rho + pi for lane (x, y): B[y + 5((2x + 3y) mod 5)] = rotl(A[x + 5y], rho[x + 5y])
-template <int I> __device__ __forceinline__ void rho_pi(uint64_t (&B)[25], const uint64_t (&A)[25])
+template <int I> __device__ __forceinline__ void rho_pi(uint32_t (&B)[25], const uint32_t (&A)[25])
 {
     constexpr int x = I % 5, y = I / 5;
-    B[y + 5 * ((2 * x + 3 * y) % 5)] = rotl<kT.rho[I]>(A[I]);
+    B[y + 5 * ((2 * x + 3 * y) % 5)] = rotl2<kT.rho[I]>(A[I], kT.rho[I] ? partner(A[I]) : 0u);
 }
 
 template <int... Is>
-__device__ __forceinline__ void rho_pi_all(uint64_t (&B)[25], const uint64_t (&A)[25],
+__device__ __forceinline__ void rho_pi_all(uint32_t (&B)[25], const uint32_t (&A)[25],
                                            std::integer_sequence<int, Is...>)
 {
     (rho_pi<Is>(B, A), ...);
 }
 
-__device__ __forceinline__ void f1600(uint64_t (&A)[25])
+__device__ __forceinline__ void f1600(uint32_t (&A)[25], int p)
 {
 #pragma unroll 1
     for (int round = 0; round < 24; round++) {
-        uint64_t C[5], B[25];
+        uint32_t C[5], Cp[5], B[25];
 #pragma unroll
         for (int x = 0; x < 5; x++) C[x] = A[x] ^ A[x + 5] ^ A[x + 10] ^ A[x + 15] ^ A[x + 20];
 #pragma unroll
+        for (int x = 0; x < 5; x++) Cp[x] = partner(C[x]);
+#pragma unroll
         for (int x = 0; x < 5; x++) {
-            const uint64_t D = C[(x + 4) % 5] ^ rotl<1>(C[(x + 1) % 5]);
+            const uint32_t D = C[(x + 4) % 5] ^ rotl2<1>(C[(x + 1) % 5], Cp[(x + 1) % 5]);
 #pragma unroll
             for (int y = 0; y < 25; y += 5) A[x + y] ^= D;
         }
@@ -100,58 +122,8 @@ __device__ __forceinline__ void f1600(uint64_t (&A)[25])
         for (int y = 0; y < 25; y += 5)
 #pragma unroll
             for (int x = 0; x < 5; x++) A[x + y] = B[x + y] ^ (~B[(x + 1) % 5 + y] & B[(x + 2) % 5 + y]);
-        A[0] ^= kRC[round];
+        A[0] ^= kRC2[2 * round + p];
     }
-}
-
-// Absorb `len` bytes (8-byte aligned source) with rate RATE and domain byte
-// `ds`, leaving the state ready for the first squeeze permutation.  Every
-// state index is a compile-time constant so the state stays in registers.
-template <int RATE> __device__ __forceinline__ void absorb(uint64_t (&A)[25], const uint8_t *in, int len, uint8_t ds)
-{
-    constexpr int RW = RATE / 8;
-#pragma unroll
-    for (int i = 0; i < 25; i++) A[i] = 0;
-    const uint64_t *w = reinterpret_cast<const uint64_t *>(in);
-    while (len >= RATE) {
-#pragma unroll
-        for (int i = 0; i < RW; i++) A[i] ^= w[i];
-        f1600(A);
-        w += RW;
-        len -= RATE;
-    }
-    const int full = len >> 3, rem = len & 7;
-    uint64_t last = (uint64_t)ds << (8 * rem);
-    const uint8_t *tail = reinterpret_cast<const uint8_t *>(w + full);
-    for (int b = 0; b < rem; b++) last |= (uint64_t)tail[b] << (8 * b);
-#pragma unroll
-    for (int i = 0; i < RW; i++) {
-        if (i < full) A[i] ^= w[i];
-        else if (i == full) A[i] ^= last;
-    }
-    A[RW - 1] ^= 0x8000000000000000ull;
-}
-
-// Absorb a 32-byte value held in four lanes.
-__device__ __forceinline__ void absorb32(uint64_t (&A)[25], const uint64_t (&m)[4], int rate, uint8_t ds)
-{
-#pragma unroll
-    for (int i = 0; i < 25; i++) A[i] = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) A[i] = m[i];
-    A[4] = ds;
-    A[rate / 8 - 1] ^= 0x8000000000000000ull;
-}
-
-// Absorb a 64-byte value held in eight lanes.
-__device__ __forceinline__ void absorb64(uint64_t (&A)[25], const uint64_t (&m)[8], int rate, uint8_t ds)
-{
-#pragma unroll
-    for (int i = 0; i < 25; i++) A[i] = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) A[i] = m[i];
-    A[8] ^= ds;
-    A[rate / 8 - 1] ^= 0x8000000000000000ull;
 }
 
 constexpr int kShake128Rate = 168;
@@ -160,76 +132,102 @@ constexpr int kSha3_512Rate = 72;
 constexpr uint8_t kShakeDS = 0x1F;
 constexpr uint8_t kSha3DS = 0x06;
 
-// SHA3-256 of an 8-byte-aligned buffer -> four lanes.
-__device__ __forceinline__ void sha3_256(uint64_t (&out)[4], const uint8_t *in, int len)
+// This lane's halves of the n 64-bit words at p_ (8-byte aligned).
+template <int N> __device__ __forceinline__ void load_words(uint32_t (&m)[N], const uint8_t *src, int p)
 {
-    uint64_t A[25];
-    absorb<kSha3_256Rate>(A, in, len, kSha3DS);
-    f1600(A);
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(src);
+#pragma unroll
+    for (int i = 0; i < N; i++) m[i] = w[2 * i + p];
+}
+template <int N> __device__ __forceinline__ void store_words(uint8_t *dst, const uint32_t (&m)[N], int p)
+{
+    uint32_t *w = reinterpret_cast<uint32_t *>(dst);
+#pragma unroll
+    for (int i = 0; i < N; i++) w[2 * i + p] = m[i];
+}
+
+// Absorb `len` bytes (8-byte aligned source) with rate RATE and domain byte
+// `ds`, leaving the state ready for the first squeeze permutation.  Every
+// state index is a compile-time constant so the state stays in registers.
+template <int RATE>
+__device__ __forceinline__ void absorb(uint32_t (&A)[25], const uint8_t *in, int len, uint8_t ds, int p)
+{
+    constexpr int RW = RATE / 8;
+#pragma unroll
+    for (int i = 0; i < 25; i++) A[i] = 0;
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(in) + p;
+    while (len >= RATE) {
+#pragma unroll
+        for (int i = 0; i < RW; i++) A[i] ^= w[2 * i];
+        f1600(A, p);
+        w += 2 * RW;
+        len -= RATE;
+    }
+    const int full = len >> 3, rem = len & 7;
+    uint64_t last = (uint64_t)ds << (8 * rem);
+    const uint8_t *tail = reinterpret_cast<const uint8_t *>(w - p + 2 * full);
+    for (int b = 0; b < rem; b++) last |= (uint64_t)tail[b] << (8 * b);
+    const uint32_t lh = p ? (uint32_t)(last >> 32) : (uint32_t)last;
+#pragma unroll
+    for (int i = 0; i < RW; i++) {
+        if (i < full) A[i] ^= w[2 * i];
+        else if (i == full) A[i] ^= lh;
+    }
+    A[RW - 1] ^= p ? 0x80000000u : 0u;
+}
+
+// Absorb N words held in registers (N * 8 bytes < rate).
+template <int N>
+__device__ __forceinline__ void absorb_words(uint32_t (&A)[25], const uint32_t (&m)[N], int rate, uint8_t ds, int p)
+{
+#pragma unroll
+    for (int i = 0; i < 25; i++) A[i] = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) A[i] = m[i];
+    A[N] ^= p ? 0u : (uint32_t)ds;
+    if (rate == kShake128Rate) A[20] ^= p ? 0x80000000u : 0u;
+    else if (rate == kSha3_256Rate) A[16] ^= p ? 0x80000000u : 0u;
+    else A[8] ^= p ? 0x80000000u : 0u;
+}
+
+// SHA3-256 of an 8-byte-aligned buffer -> four words.
+__device__ __forceinline__ void sha3_256(uint32_t (&out)[4], const uint8_t *in, int len, int p)
+{
+    uint32_t A[25];
+    absorb<kSha3_256Rate>(A, in, len, kSha3DS, p);
+    f1600(A, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = A[i];
 }
 
-__device__ __forceinline__ void sha3_256_32(uint64_t (&out)[4], const uint64_t (&m)[4])
+template <int N> __device__ __forceinline__ void sha3_256_words(uint32_t (&out)[4], const uint32_t (&m)[N], int p)
 {
-    uint64_t A[25];
-    absorb32(A, m, kSha3_256Rate, kSha3DS);
-    f1600(A);
+    uint32_t A[25];
+    absorb_words(A, m, kSha3_256Rate, kSha3DS, p);
+    f1600(A, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = A[i];
 }
 
-__device__ __forceinline__ void sha3_256_64(uint64_t (&out)[4], const uint64_t (&m)[8])
+__device__ __forceinline__ void sha3_512_words(uint32_t (&out)[8], const uint32_t (&m)[8], int p)
 {
-    uint64_t A[25];
-    absorb64(A, m, kSha3_256Rate, kSha3DS);
-    f1600(A);
-#pragma unroll
-    for (int i = 0; i < 4; i++) out[i] = A[i];
-}
-
-__device__ __forceinline__ void sha3_512_64(uint64_t (&out)[8], const uint64_t (&m)[8])
-{
-    uint64_t A[25];
-    absorb64(A, m, kSha3_512Rate, kSha3DS);
-    f1600(A);
+    uint32_t A[25];
+    absorb_words(A, m, kSha3_512Rate, kSha3DS, p);
+    f1600(A, p);
 #pragma unroll
     for (int i = 0; i < 8; i++) out[i] = A[i];
 }
 
-// Squeeze the sponge (absorbed, not yet permuted) as a stream of `count`
-// w-bit values, w <= 16, least significant bit first, calling sink(k, v) for
-// value k.  Blocks are consumed as 42 half-lanes in a fully unrolled loop, so
-// no state word is indexed at run time.
-template <class Sink> __device__ __forceinline__ void squeeze_values(uint64_t (&A)[25], int w, int count, Sink &&sink)
+// Squeeze `nwords` 64-bit words of output (absorbed, not yet permuted state)
+// to dst, each lane writing its halves.
+__device__ __forceinline__ void squeeze_words(uint32_t (&A)[25], uint8_t *dst, int nwords, int p)
 {
-    const uint32_t mask = (1u << w) - 1u;
-    uint64_t buf = 0;
-    int nbits = 0, k = 0;
-    while (k < count) {
-        f1600(A);
-#pragma unroll
-        for (int h = 0; h < kShake128Rate / 4; h++) {
-            buf |= (uint64_t)(uint32_t)(A[h >> 1] >> (32 * (h & 1))) << nbits;
-            nbits += 32;
-            while (nbits >= w && k < count) {
-                sink(k, (uint32_t)buf & mask);
-                buf >>= w;
-                nbits -= w;
-                k++;
-            }
-        }
-    }
-}
-
-// Squeeze `nwords` 64-bit words of output (absorbed, not yet permuted state).
-__device__ __forceinline__ void squeeze_words(uint64_t (&A)[25], uint64_t *dst, int nwords)
-{
+    uint32_t *d = reinterpret_cast<uint32_t *>(dst) + p;
     for (int k = 0; k < nwords; k += kShake128Rate / 8) {
-        f1600(A);
+        f1600(A, p);
 #pragma unroll
         for (int i = 0; i < kShake128Rate / 8; i++)
-            if (k + i < nwords) dst[k + i] = A[i];
+            if (k + i < nwords) d[2 * (k + i)] = A[i];
     }
 }
 

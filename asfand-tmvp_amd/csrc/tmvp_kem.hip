@@ -12,9 +12,8 @@
 // absent from the reference and follows Saber round 3 (seed_A is hashed with
 // SHAKE-128, b = round(A^T s)).
 //
-// Sponges run one thread per instance (state in registers); codecs run one
-// thread per group of 8 coefficients, which is exactly w bytes of a w-bit
-// stream (LSB first, as every reference codec).
+// Sponges run on a pair of lanes per instance (state in registers, split in
+// 32-bit halves); codecs and CBD sampling run one thread per 64 coefficients.
 #include <string.h>
 
 #include <utility>
@@ -28,169 +27,165 @@ namespace tmvp {
 using namespace keccak;
 
 // ---------------------------------------------------------------- sampling
+//
+// Every sponge runs on a pair of adjacent lanes (tmvp_keccak.h): thread g of a
+// sponge kernel works on pair t = g >> 1, half p = g & 1, and both lanes of a
+// pair always take the same branch.
 
-__device__ __forceinline__ void load32(uint64_t (&m)[4], const uint8_t *p)
-{
-    const uint64_t *w = reinterpret_cast<const uint64_t *>(p);
-#pragma unroll
-    for (int i = 0; i < 4; i++) m[i] = w[i];
-}
-__device__ __forceinline__ void store32(uint8_t *p, const uint64_t (&m)[4])
-{
-    uint64_t *w = reinterpret_cast<uint64_t *>(p);
-#pragma unroll
-    for (int i = 0; i < 4; i++) w[i] = m[i];
-}
+#define TMVP_PAIR(t0)                                                                                                \
+    const int64_t t = (((int64_t)blockIdx.x * 64 + threadIdx.x) >> 1) + (t0);                                       \
+    const int p = (int)(threadIdx.x & 1)
 
-// Collects 16-bit values four at a time into 8-byte stores.
-struct Quad {
-    uint2 *dst;
-    uint64_t acc = 0;
-    __device__ __forceinline__ void put(int k, uint32_t v)
-    {
-        acc |= (uint64_t)v << (16 * (k & 3));
-        if ((k & 3) == 3) {
-            dst[k >> 2] = make_uint2((uint32_t)acc, (uint32_t)(acc >> 32));
-            acc = 0;
-        }
-    }
-};
-
-// GenMatrix, first half: the SHAKE-128(seed) stream of L*L*EQ*32 bytes, which
-// the EQ-bit unpacker turns into L*L polynomials, row-major
-// (S/src/pack.cu:145-153, SB/src/pack.cu:41-50).  Keeping the sponge to
-// permute-and-store leaves the unpacking to a kernel with one thread per 64
-// coefficients.
-__device__ __forceinline__ void gen_matrix_stream(const KemParams &P, const uint64_t (&seed)[4], uint8_t *dst)
+// The SHAKE-128(seed) stream of `bytes` bytes to dst: GenMatrix
+// (S/src/pack.cu:145-153, SB/src/pack.cu:41-50) and GenSecret
+// (S/src/poly.cu:1062-1073, SB/src/fips202.cu:655-668) unpack it afterwards
+// with kernels of one thread per 64 coefficients.
+__device__ __forceinline__ void shake_stream(const uint32_t (&seed)[4], uint8_t *dst, int bytes, int p)
 {
-    uint64_t A[25];
-    absorb32(A, seed, kShake128Rate, kShakeDS);
-    squeeze_words(A, reinterpret_cast<uint64_t *>(dst), P.araw / 8);
+    uint32_t A[25];
+    absorb_words(A, seed, kShake128Rate, kShakeDS, p);
+    squeeze_words(A, dst, bytes / 8, p);
 }
 
-// GenSecret: SHAKE-128(seed) read as L*256 mu-bit groups, each the popcount of
-// its low half minus that of its high half (S/src/poly.cu:836-916, :1062-1073;
-// SB/src/fips202.cu:625-668); Saber stores int16, Sable the value mod q.
-__device__ __forceinline__ void gen_secret(const KemParams &P, const uint64_t (&seed)[4], uint16_t *dst)
+// G(m | H(pk)) -> buf = m | H(pk), kr; returns the coins kr[32:64]; the
+// shared first step of encapsulation and of the re-encryption in
+// decapsulation (S/src/cuda_kernel.cu:116-118, :378-379).
+__device__ __forceinline__ void hash_g(const uint32_t (&m)[4], const uint32_t (&hpk)[4], uint8_t *buf, uint8_t *kr,
+                                       uint32_t (&coins)[4], int p)
 {
-    uint64_t A[25];
-    absorb32(A, seed, kShake128Rate, kShakeDS);
-    const int half = P.mu / 2;
-    const uint32_t lmask = (1u << half) - 1u, qmask = P.sable ? (1u << P.eq) - 1u : 0xffffu;
-    Quad q{reinterpret_cast<uint2 *>(dst)};
-    squeeze_values(A, P.mu, P.l * kN, [&](int k, uint32_t bits) {
-        const int v = __builtin_popcount(bits & lmask) - __builtin_popcount(bits >> half);
-        q.put(k, (uint32_t)v & qmask);
-    });
-}
-
-// H(pk) | m -> G -> kr; the shared first step of encapsulation and of the
-// re-encryption in decapsulation (S/src/cuda_kernel.cu:116-118, :378-379).
-__device__ __forceinline__ void hash_g(const uint64_t (&m)[4], const uint64_t (&hpk)[4], uint8_t *buf, uint8_t *kr,
-                                       uint64_t (&coins)[4])
-{
-    uint64_t b[8], k[8];
+    uint32_t b[8], k[8];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         b[i] = m[i];
         b[4 + i] = hpk[i];
     }
-    sha3_512_64(k, b);
-    uint64_t *bo = reinterpret_cast<uint64_t *>(buf);
-    uint64_t *ko = reinterpret_cast<uint64_t *>(kr);
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        bo[i] = b[i];
-        ko[i] = k[i];
-    }
+    sha3_512_words(k, b, p);
+    store_words(buf, b, p);
+    store_words(kr, k, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) coins[i] = k[4 + i];
 }
 
-// The front kernels run the long matrix sponge (threads [0, B)) beside the
-// rest of the instance's hashing and secret sampling (threads [B, 2B)); the
-// matrix stream is unpacked afterwards.
+// The front kernels run the long matrix sponge (pairs [0, B)) beside the rest
+// of the instance's hashing and its secret stream (pairs [B, 2B)); a kernel
+// starting at pair t0 = B skips the matrix.
 
 // Key generation: seeds[t] = seed_A | seed_s | z.  seed_A' = SHAKE-128(seed_A)
-// goes to the pk tail, A = GenMatrix(seed_A'); s = GenSecret(seed_s); z goes
-// to the sk tail.
+// goes to the pk tail and seeds the matrix stream; seed_s seeds the secret
+// stream; z goes to the sk tail.
 __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
-                                                      uint8_t *araw, uint16_t *S, int64_t B)
+                                                      uint8_t *araw, uint8_t *sraw, int64_t B)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    TMVP_PAIR(0);
     if (t >= 2 * B) return;
-    uint64_t m[4];
+    uint32_t m[4];
     if (t < B) {
-        uint64_t st[25];
-        load32(m, seeds + t * 96);
-        absorb32(st, m, kShake128Rate, kShakeDS);
-        f1600(st);
-        const uint64_t sa[4] = {st[0], st[1], st[2], st[3]};
-        store32(pk + t * P.pk + P.pvc, sa);
-        gen_matrix_stream(P, sa, araw + t * P.araw);
+        uint32_t st[25];
+        load_words(m, seeds + t * 96, p);
+        absorb_words(st, m, kShake128Rate, kShakeDS, p);
+        f1600(st, p);
+        const uint32_t sa[4] = {st[0], st[1], st[2], st[3]};
+        store_words(pk + t * P.pk + P.pvc, sa, p);
+        shake_stream(sa, araw + t * P.araw, P.araw, p);
     } else {
         const int64_t i = t - B;
-        load32(m, seeds + i * 96 + 32);
-        gen_secret(P, m, S + i * (P.l * kN));
-        load32(m, seeds + i * 96 + 64);
-        store32(sk + i * P.sk + P.sk - 32, m);
+        load_words(m, seeds + i * 96 + 32, p);
+        shake_stream(m, sraw + i * P.sraw, P.sraw, p);
+        load_words(m, seeds + i * 96 + 64, p);
+        store_words(sk + i * P.sk + P.sk - 32, m, p);
     }
 }
 
-// Encapsulation: A from the pk tail; m = H(rnd), kr = G(m | H(pk)), s' =
-// GenSecret(kr[32:64]).  Thread t takes role t + t0: t0 = B skips the matrix
-// (already resident), and hpk (if not null) holds H(pk) per instance.
+// Encapsulation: matrix stream from the pk tail; m = H(rnd), kr = G(m | H(pk)),
+// secret stream from kr[32:64].  hpk (if not null) holds H(pk) per instance.
 __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
                                                        const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint8_t *araw,
-                                                       uint16_t *S, int64_t B, int64_t t0)
+                                                       uint8_t *sraw, int64_t B, int64_t t0)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x + t0;
+    TMVP_PAIR(t0);
     if (t >= 2 * B) return;
-    uint64_t m[4];
+    uint32_t m[4];
     if (t < B) {
-        load32(m, pk + t * P.pk + P.pvc);
-        gen_matrix_stream(P, m, araw + t * P.araw);
+        load_words(m, pk + t * P.pk + P.pvc, p);
+        shake_stream(m, araw + t * P.araw, P.araw, p);
     } else {
         const int64_t i = t - B;
-        uint64_t r[4], h[4], coins[4];
-        load32(r, rnd + i * 32);
-        sha3_256_32(m, r);
-        if (hpk) load32(h, hpk + i * 32);
-        else sha3_256(h, pk + i * P.pk, P.pk);
-        hash_g(m, h, buf + i * 64, kr + i * 64, coins);
-        gen_secret(P, coins, S + i * (P.l * kN));
+        uint32_t r[4], h[4], coins[4];
+        load_words(r, rnd + i * 32, p);
+        sha3_256_words(m, r, p);
+        if (hpk) load_words(h, hpk + i * 32, p);
+        else sha3_256(h, pk + i * P.pk, P.pk, p);
+        hash_g(m, h, buf + i * 64, kr + i * 64, coins, p);
+        shake_stream(coins, sraw + i * P.sraw, P.sraw, p);
     }
 }
 
-// Decapsulation, after the decryption: A from the pk inside the sk; m' from
-// its bits, kr = G(m' | H(pk)) with H(pk) from the sk, s' = GenSecret(kr[32:64]).
+// Decapsulation, after the decryption: matrix stream from the pk inside the
+// sk; m' from its bits, kr = G(m' | H(pk)) with H(pk) from the sk, secret
+// stream from kr[32:64].
 __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
-                                                       uint8_t *buf, uint8_t *kr, uint8_t *araw, uint16_t *S,
+                                                       uint8_t *buf, uint8_t *kr, uint8_t *araw, uint8_t *sraw,
                                                        int64_t B, int64_t t0)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x + t0;
+    TMVP_PAIR(t0);
     if (t >= 2 * B) return;
-    uint64_t m[4];
+    uint32_t m[4];
     if (t < B) {
-        load32(m, sk + t * P.sk + P.indcpa_sk + P.pvc);
-        gen_matrix_stream(P, m, araw + t * P.araw);
+        load_words(m, sk + t * P.sk + P.indcpa_sk + P.pvc, p);
+        shake_stream(m, araw + t * P.araw, P.araw, p);
     } else {
         const int64_t i = t - B;
-        const uint32_t *mb = reinterpret_cast<const uint32_t *>(mbits + i * kN); // bit k in uint16 k
+        // this lane's 32 bits of each message word: bits 64w + 32p + j
+        const uint32_t *mb = reinterpret_cast<const uint32_t *>(mbits + i * kN + 32 * p); // bit k in uint16 k
 #pragma unroll
         for (int w = 0; w < 4; w++) {
-            uint64_t x = 0;
+            uint32_t x = 0;
 #pragma unroll
-            for (int j = 0; j < 32; j++) {
+            for (int j = 0; j < 16; j++) {
                 const uint32_t two = mb[32 * w + j];
-                x |= (uint64_t)(two & 1u) << (2 * j) | (uint64_t)((two >> 16) & 1u) << (2 * j + 1);
+                x |= ((two & 1u) << (2 * j)) | (((two >> 16) & 1u) << (2 * j + 1));
             }
             m[w] = x;
         }
-        uint64_t h[4], coins[4];
-        load32(h, sk + i * P.sk + P.sk - 64);
-        hash_g(m, h, buf + i * 64, kr + i * 64, coins);
-        gen_secret(P, coins, S + i * (P.l * kN));
+        uint32_t h[4], coins[4];
+        load_words(h, sk + i * P.sk + P.sk - 64, p);
+        hash_g(m, h, buf + i * 64, kr + i * 64, coins, p);
+        shake_stream(coins, sraw + i * P.sraw, P.sraw, p);
+    }
+}
+
+// CBD_mu over the secret stream (S/src/poly.cu:836-916; SB/src/fips202.cu:625-651):
+// coefficient k is the popcount of bits [k*mu, k*mu + mu/2) minus that of the
+// next mu/2 bits; one thread per 64 coefficients, which is exactly mu words.
+// Saber stores the value as int16, Sable masked mod q.
+template <int MU>
+__global__ __launch_bounds__(256) void cbd_kernel(const uint8_t *sraw, int64_t sraw_stride, uint16_t *S,
+                                                  int64_t s_stride, int chunks, uint32_t qmask, int64_t B)
+{
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= B * chunks) return;
+    const int64_t t = gid / chunks;
+    const int c = (int)(gid - t * chunks);
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(sraw + t * sraw_stride + (int64_t)c * MU * 8);
+    uint64_t words[MU];
+#pragma unroll
+    for (int k = 0; k < MU; k++) words[k] = src[k];
+    constexpr int H = MU / 2;
+    constexpr uint32_t lo = (1u << H) - 1u;
+    uint4 *d = reinterpret_cast<uint4 *>(S + t * s_stride + c * 64);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        uint32_t e[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int bit = (8 * q + j) * MU, wi = bit >> 6, o = bit & 63;
+            uint64_t x = words[wi] >> o;
+            if (o + MU > 64) x |= words[wi + 1] << (64 - o);
+            const uint32_t bits = (uint32_t)x;
+            const int v = __builtin_popcount(bits & lo) - __builtin_popcount((bits >> H) & lo);
+            e[j] = (uint32_t)v & qmask;
+        }
+        d[q] = make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16), e[6] | (e[7] << 16));
     }
 }
 
@@ -290,54 +285,54 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint8_t *src, int64_t
         reinterpret_cast<const uint64_t *>(src + (int64_t)keys[t] * src_stride)[k];
 }
 
-// ---------------------------------------------------------------- hashing steps (one thread per instance)
+// ---------------------------------------------------------------- hashing steps (one lane pair per instance)
 
 // H(pk) of each public key (for a key store).
 __global__ __launch_bounds__(64) void hash_pk_kernel(KemParams P, const uint8_t *pk, uint8_t *hpk, int64_t B)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    TMVP_PAIR(0);
     if (t >= B) return;
-    uint64_t h[4];
-    sha3_256(h, pk + t * P.pk, P.pk);
-    store32(hpk + t * 32, h);
+    uint32_t h[4];
+    sha3_256(h, pk + t * P.pk, P.pk, p);
+    store_words(hpk + t * 32, h, p);
 }
 
 // The GenMatrix stream of each public key's seed (for a key store).
 __global__ __launch_bounds__(64) void matrix_stream_kernel(KemParams P, const uint8_t *pk, uint8_t *araw, int64_t B)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    TMVP_PAIR(0);
     if (t >= B) return;
-    uint64_t m[4];
-    load32(m, pk + t * P.pk + P.pvc);
-    gen_matrix_stream(P, m, araw + t * P.araw);
+    uint32_t m[4];
+    load_words(m, pk + t * P.pk + P.pvc, p);
+    shake_stream(m, araw + t * P.araw, P.araw, p);
 }
 
 // Key generation, last step: H(pk) into the sk.
 __global__ __launch_bounds__(64) void kp_hash_kernel(KemParams P, const uint8_t *pk, uint8_t *sk, int64_t B)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    TMVP_PAIR(0);
     if (t >= B) return;
-    uint64_t h[4];
-    sha3_256(h, pk + t * P.pk, P.pk);
-    store32(sk + t * P.sk + P.sk - 64, h);
+    uint32_t h[4];
+    sha3_256(h, pk + t * P.pk, P.pk, p);
+    store_words(sk + t * P.sk + P.sk - 64, h, p);
 }
 
 // Encapsulation, last step (S/src/cuda_kernel.cu:185-186): kr[32:] = H(ct), ss = H(kr).
 __global__ __launch_bounds__(64) void enc_hash2_kernel(KemParams P, const uint8_t *ct, uint8_t *kr, uint8_t *ss,
                                                        int64_t B)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    TMVP_PAIR(0);
     if (t >= B) return;
-    uint64_t h[4], k[8], s[4];
-    sha3_256(h, ct + t * P.ct, P.ct);
-    const uint64_t *ki = reinterpret_cast<const uint64_t *>(kr + t * 64);
+    uint32_t h[4], k[8], s[4], k0[4];
+    sha3_256(h, ct + t * P.ct, P.ct, p);
+    load_words(k0, kr + t * 64, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        k[i] = ki[i];
+        k[i] = k0[i];
         k[4 + i] = h[i];
     }
-    sha3_256_64(s, k);
-    store32(ss + t * 32, s);
+    sha3_256_words(s, k, p);
+    store_words(ss + t * 32, s, p);
 }
 
 // Decapsulation, last step: compare ct with the re-encryption, kr[32:] = H(ct),
@@ -346,25 +341,26 @@ __global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint8_t 
                                                      const uint8_t *sk, uint8_t *kr, uint8_t *ss, uint8_t *fail,
                                                      int64_t B)
 {
-    const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    TMVP_PAIR(0);
     if (t >= B) return;
-    const uint64_t *a = reinterpret_cast<const uint64_t *>(ct + t * P.ct);
-    const uint64_t *b = reinterpret_cast<const uint64_t *>(ctp + t * P.ct);
-    uint64_t diff = 0;
-    for (int i = 0; i < P.ct / 8; i++) diff |= a[i] ^ b[i];
-    const uint64_t mask = 0ull - (uint64_t)(diff != 0); // all ones on mismatch
-    uint64_t h[4], k[8], s[4];
-    sha3_256(h, ct + t * P.ct, P.ct);
-    const uint64_t *ki = reinterpret_cast<const uint64_t *>(kr + t * 64);
-    const uint64_t *z = reinterpret_cast<const uint64_t *>(sk + t * P.sk + P.sk - 32);
+    const uint32_t *a = reinterpret_cast<const uint32_t *>(ct + t * P.ct) + p;
+    const uint32_t *b = reinterpret_cast<const uint32_t *>(ctp + t * P.ct) + p;
+    uint32_t diff = 0;
+    for (int i = 0; i < P.ct / 8; i++) diff |= a[2 * i] ^ b[2 * i];
+    diff |= partner(diff);
+    const uint32_t mask = 0u - (uint32_t)(diff != 0); // all ones on mismatch, in both lanes
+    uint32_t h[4], k[8], s[4], k0[4], z[4];
+    sha3_256(h, ct + t * P.ct, P.ct, p);
+    load_words(k0, kr + t * 64, p);
+    load_words(z, sk + t * P.sk + P.sk - 32, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        k[i] = ki[i] ^ (mask & (ki[i] ^ z[i]));
+        k[i] = k0[i] ^ (mask & (k0[i] ^ z[i]));
         k[4 + i] = h[i];
     }
-    sha3_256_64(s, k);
-    store32(ss + t * 32, s);
-    if (fail) fail[t] = (uint8_t)(mask & 1u);
+    sha3_256_words(s, k, p);
+    store_words(ss + t * 32, s, p);
+    if (fail && p == 0) fail[t] = (uint8_t)(mask & 1u);
 }
 
 // ---------------------------------------------------------------- host chains
@@ -451,6 +447,31 @@ static hipError_t unpack(const uint8_t *src, int64_t src_stride, int src_off, ui
     return unpack_w<1>(src, src_stride, src_off, dst, dst_stride, w, ncoef / 64, xf, B, st);
 }
 
+// CBD_mu over the secret streams of B instances into W.s.
+template <int MU>
+static hipError_t cbd_launch(const KemParams &P, const KemWorkspace &W, int64_t B, hipStream_t st)
+{
+    const int chunks = P.l * kN / 64;
+    const uint32_t qmask = P.sable ? (1u << P.eq) - 1u : 0xffffu;
+    hipLaunchKernelGGL((cbd_kernel<MU>), dim3(blocks_for(B * chunks, 256)), dim3(256), 0, st, W.sraw,
+                       (int64_t)P.sraw, W.s, (int64_t)(P.l * kN), chunks, qmask, B);
+    return hipGetLastError();
+}
+
+static hipError_t gen_secret(const KemParams &P, const KemWorkspace &W, int64_t B, hipStream_t st)
+{
+    switch (P.mu) {
+    case 2: return cbd_launch<2>(P, W, B, st);
+    case 6: return cbd_launch<6>(P, W, B, st);
+    case 8: return cbd_launch<8>(P, W, B, st);
+    case 10: return cbd_launch<10>(P, W, B, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+// Threads for n lane pairs.
+static inline unsigned pair_blocks(int64_t n) { return blocks_for(2 * n, 64); }
+
 // IND-CPA encryption, after sampling: A and s' are in the workspace, the
 // message in buf[t][0:32] (S/src/cuda_kernel.cu:124-178).  ct gets
 // pack_EP(round(A s')) | pack_ET(round(b . s' - m)).
@@ -472,10 +493,11 @@ hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, c
 {
     if (B <= 0) return hipSuccess;
     const int L = P.l;
-    hipLaunchKernelGGL(kp_front_kernel, dim3(blocks_for(2 * B, 64)), dim3(64), 0, st, P, seeds, pk, sk, W.araw, W.s,
+    hipLaunchKernelGGL(kp_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk, W.araw, W.sraw,
                        B);
     TRY(hipGetLastError());
     TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
+    TRY(gen_secret(P, W, B, st));
     TRY(product(P, backend, kMatVecT, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
     TRY(pack(W.v1, L * kN, pk, P.pk, 0, P.ep, L * kN, kXfNone, B, st));
     if (P.sable)
@@ -485,7 +507,7 @@ hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, c
     hipLaunchKernelGGL(copy_kernel, dim3(blocks_for(B * (P.pk / 8), 256)), dim3(256), 0, st, pk, (int64_t)P.pk, 0, sk,
                        (int64_t)P.sk, P.indcpa_sk, P.pk, B);
     TRY(hipGetLastError());
-    hipLaunchKernelGGL(kp_hash_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, pk, sk, B);
+    hipLaunchKernelGGL(kp_hash_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, pk, sk, B);
     return hipGetLastError();
 }
 
@@ -493,12 +515,13 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
                       uint8_t *ct, uint8_t *ss, int64_t B, int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(enc_front_kernel, dim3(blocks_for(2 * B, 64)), dim3(64), 0, st, P, rnd, pk, nullptr, W.buf,
-                       W.kr, W.araw, W.s, B, (int64_t)0);
+    hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk, nullptr, W.buf, W.kr,
+                       W.araw, W.sraw, B, (int64_t)0);
     TRY(hipGetLastError());
     TRY(unpack(W.araw, P.araw, 0, W.A, P.l * P.l * kN, P.eq, P.l * P.l * kN, kXfNone, B, st));
+    TRY(gen_secret(P, W, B, st));
     TRY(indcpa_enc_tail(P, W, backend, pk, P.pk, ct, B, status, st));
-    hipLaunchKernelGGL(enc_hash2_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, ct, W.kr, ss, B);
+    hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
 }
 
@@ -519,12 +542,13 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     TRY(product(P, backend, kInner, kEpiDec, W.poly, W.v2, W.s, W.bits, B, status, st));
     // re-encryption of m' under the pk stored in the sk
     const int64_t t0 = resident ? B : 0;
-    hipLaunchKernelGGL(dec_front_kernel, dim3(blocks_for(2 * B - t0, 64)), dim3(64), 0, st, P, W.poly, sk, W.buf,
-                       W.kr, W.araw, W.s, B, t0);
+    hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(2 * B - t0)), dim3(64), 0, st, P, W.poly, sk, W.buf, W.kr,
+                       W.araw, W.sraw, B, t0);
     TRY(hipGetLastError());
     if (!resident) TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
+    TRY(gen_secret(P, W, B, st));
     TRY(indcpa_enc_tail(P, W, backend, sk + P.indcpa_sk, P.sk, W.ctp, B, status, st));
-    hipLaunchKernelGGL(dec_fin_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, ct, W.ctp, sk, W.kr, ss, fail, B);
+    hipLaunchKernelGGL(dec_fin_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.ctp, sk, W.kr, ss, fail, B);
     return hipGetLastError();
 }
 
@@ -554,11 +578,12 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
     TRY(gather(reinterpret_cast<const uint8_t *>(K.A), abytes, keys, reinterpret_cast<uint8_t *>(W.A), abytes, abytes,
                B, st));
     TRY(gather(K.hpk, 32, keys, W.hpkg, 32, 32, B, st));
-    hipLaunchKernelGGL(enc_front_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg, W.buf, W.kr,
-                       W.araw, W.s, B, B);
+    hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg, W.buf, W.kr,
+                       W.araw, W.sraw, B, B);
     TRY(hipGetLastError());
+    TRY(gen_secret(P, W, B, st));
     TRY(indcpa_enc_tail(P, W, backend, W.pkg, P.pk, ct, B, status, st));
-    hipLaunchKernelGGL(enc_hash2_kernel, dim3(blocks_for(B, 64)), dim3(64), 0, st, P, ct, W.kr, ss, B);
+    hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
 }
 
@@ -578,10 +603,10 @@ hipError_t kem_store_derive(const KemParams &P, const uint8_t *pk, int64_t n, ui
                             uint8_t *araw, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(matrix_stream_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, st, P, pk, araw, n);
+    hipLaunchKernelGGL(matrix_stream_kernel, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk, araw, n);
     TRY(hipGetLastError());
     TRY(unpack(araw, P.araw, 0, A, P.l * P.l * kN, P.eq, P.l * P.l * kN, kXfNone, n, st));
-    hipLaunchKernelGGL(hash_pk_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, st, P, pk, hpk, n);
+    hipLaunchKernelGGL(hash_pk_kernel, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk, hpk, n);
     return hipGetLastError();
 }
 
@@ -602,6 +627,7 @@ size_t kem_workspace_bytes(const KemParams &P, int64_t B)
     b += n * (size_t)P.pk;   // pkg
     b += n * (size_t)P.sk;   // skg
     b += n * 32;             // hpkg
+    b += n * (size_t)P.sraw; // sraw
     return b + 16 * 256;
 }
 
@@ -626,6 +652,7 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
     W.pkg = take(n * (size_t)P.pk);
     W.skg = take(n * (size_t)P.sk);
     W.hpkg = take(n * 32);
+    W.sraw = take(n * (size_t)P.sraw);
 }
 
 } // namespace tmvp
