@@ -16,7 +16,9 @@ struct Workspace {
     hipStream_t stream;
     void *ptr;
     size_t bytes;
-    uint64_t used; // last-use stamp
+    uint64_t used;     // last-use stamp
+    hipStream_t aux;   // side stream for decapsulation steps that depend only on inputs
+    hipEvent_t fork, join;
 };
 // A measured plan for the multi-job MFMA launch of one work-list shape.
 constexpr int kMaxPlans = 32;
@@ -116,37 +118,63 @@ bool capturing(hipStream_t stream)
 // Workspace of at least `bytes` for `stream`; growing it synchronises the
 // device.  A stream seen for the first time when all slots are taken gets the
 // least recently used slot, after a device synchronisation.
-int workspace(tmvp_ctx *ctx, hipStream_t stream, size_t bytes, uint8_t **out)
+Workspace *workspace(tmvp_ctx *ctx, hipStream_t stream, size_t bytes, int *rc)
 {
     Workspace *w = nullptr;
+    *rc = TMVP_OK;
     for (int i = 0; i < ctx->nws; i++)
         if (ctx->ws[i].stream == stream) w = &ctx->ws[i];
     if (!w) {
-        if (capturing(stream)) return TMVP_EUNSUPPORTED; // reserve before capturing
+        if (capturing(stream)) { // reserve before capturing
+            *rc = TMVP_EUNSUPPORTED;
+            return nullptr;
+        }
         if (ctx->nws < kMaxWorkspaces) {
-            w = &ctx->ws[ctx->nws++];
+            w = &ctx->ws[ctx->nws];
             w->ptr = nullptr;
             w->bytes = 0;
+            if (hipStreamCreateWithFlags(&w->aux, hipStreamNonBlocking) != hipSuccess) {
+                *rc = TMVP_EHIP;
+                return nullptr;
+            }
+            if (hipEventCreateWithFlags(&w->fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&w->join, hipEventDisableTiming) != hipSuccess) {
+                (void)hipStreamDestroy(w->aux);
+                *rc = TMVP_EHIP;
+                return nullptr;
+            }
+            ctx->nws++;
         } else {
             w = &ctx->ws[0];
             for (int i = 1; i < kMaxWorkspaces; i++)
                 if (ctx->ws[i].used < w->used) w = &ctx->ws[i];
-            if (hipDeviceSynchronize() != hipSuccess) return TMVP_EHIP;
+            if (hipDeviceSynchronize() != hipSuccess) {
+                *rc = TMVP_EHIP;
+                return nullptr;
+            }
         }
         w->stream = stream;
     }
     w->used = ++ctx->ws_clock;
     if (w->bytes < bytes) {
-        if (capturing(stream)) return TMVP_EUNSUPPORTED; // no allocation inside a graph capture
-        if (hipDeviceSynchronize() != hipSuccess) return TMVP_EHIP;
+        if (capturing(stream)) { // no allocation inside a graph capture
+            *rc = TMVP_EUNSUPPORTED;
+            return nullptr;
+        }
+        if (hipDeviceSynchronize() != hipSuccess) {
+            *rc = TMVP_EHIP;
+            return nullptr;
+        }
         if (w->ptr) (void)hipFree(w->ptr);
         w->ptr = nullptr;
         w->bytes = 0;
-        if (hipMalloc(&w->ptr, bytes) != hipSuccess) return TMVP_ENOMEM;
+        if (hipMalloc(&w->ptr, bytes) != hipSuccess) {
+            *rc = TMVP_ENOMEM;
+            return nullptr;
+        }
         w->bytes = bytes;
     }
-    *out = static_cast<uint8_t *>(w->ptr);
-    return TMVP_OK;
+    return w;
 }
 
 bool kem_args_ok(const tmvp_ctx *ctx, int ps, int backend, int64_t batch)
@@ -160,10 +188,12 @@ int kem_setup(tmvp_ctx *ctx, int ps, int64_t batch, tmvp_stream_t stream, KemPar
     int rc = set_device(ctx);
     if (rc) return rc;
     P = kem_params(ps);
-    uint8_t *base = nullptr;
-    rc = workspace(ctx, reinterpret_cast<hipStream_t>(stream), kem_workspace_bytes(P, batch), &base);
-    if (rc) return rc;
-    kem_workspace_carve(P, batch, base, W);
+    Workspace *w = workspace(ctx, reinterpret_cast<hipStream_t>(stream), kem_workspace_bytes(P, batch), &rc);
+    if (!w) return rc;
+    kem_workspace_carve(P, batch, static_cast<uint8_t *>(w->ptr), W);
+    W.aux = w->aux;
+    W.fork = w->fork;
+    W.join = w->join;
     return TMVP_OK;
 }
 
@@ -423,8 +453,12 @@ int tmvp_destroy(tmvp_ctx *ctx)
     if (!ctx) return TMVP_EARG;
     int rc = set_device(ctx);
     if (hipDeviceSynchronize() != hipSuccess) rc = TMVP_EHIP;
-    for (int i = 0; i < ctx->nws; i++)
+    for (int i = 0; i < ctx->nws; i++) {
         if (ctx->ws[i].ptr) (void)hipFree(ctx->ws[i].ptr);
+        (void)hipEventDestroy(ctx->ws[i].fork);
+        (void)hipEventDestroy(ctx->ws[i].join);
+        (void)hipStreamDestroy(ctx->ws[i].aux);
+    }
     if (ctx->d_status) (void)hipFree(ctx->d_status);
     delete ctx;
     return rc;

@@ -23,6 +23,12 @@ struct KemWorkspace {
     uint16_t *A, *s, *v1, *v2, *bits, *poly;
     uint8_t *buf, *kr, *ctp, *araw, *sraw;
     uint8_t *pkg, *skg, *hpkg; // key material gathered from a key store
+    uint8_t *hct;              // H(ct) per instance (decapsulation)
+    // Decapsulation runs the steps that depend only on its inputs (the matrix
+    // stream, H(ct)) on `aux`, forked from and joined to the caller's stream
+    // with the two events.  Owned by whoever owns the workspace.
+    hipStream_t aux;
+    hipEvent_t fork, join;
 };
 
 // Device-resident keys of one parameter set: pk [n][pk], sk [n][sk] (may be

@@ -120,38 +120,50 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
     }
 }
 
-// Decapsulation, after the decryption: matrix stream from the pk inside the
-// sk; m' from its bits, kr = G(m' | H(pk)) with H(pk) from the sk, secret
-// stream from kr[32:64].
-__global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
-                                                       uint8_t *buf, uint8_t *kr, uint8_t *araw, uint8_t *sraw,
-                                                       int64_t B, int64_t t0)
+// Decapsulation, before the decryption and beside it (on the auxiliary
+// stream): the matrix stream from the pk inside the sk (pairs [t0, B); t0 = B
+// when A is resident) and H(ct) (pairs [B, 2B)).
+__global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t *ct, const uint8_t *sk, uint8_t *araw,
+                                                     uint8_t *hct, int64_t B, int64_t t0)
 {
     TMVP_PAIR(t0);
     if (t >= 2 * B) return;
-    uint32_t m[4];
     if (t < B) {
+        uint32_t m[4];
         load_words(m, sk + t * P.sk + P.indcpa_sk + P.pvc, p);
         shake_stream(m, araw + t * P.araw, P.araw, p);
     } else {
         const int64_t i = t - B;
-        // this lane's 32 bits of each message word: bits 64w + 32p + j
-        const uint32_t *mb = reinterpret_cast<const uint32_t *>(mbits + i * kN + 32 * p); // bit k in uint16 k
-#pragma unroll
-        for (int w = 0; w < 4; w++) {
-            uint32_t x = 0;
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const uint32_t two = mb[32 * w + j];
-                x |= ((two & 1u) << (2 * j)) | (((two >> 16) & 1u) << (2 * j + 1));
-            }
-            m[w] = x;
-        }
-        uint32_t h[4], coins[4];
-        load_words(h, sk + i * P.sk + P.sk - 64, p);
-        hash_g(m, h, buf + i * 64, kr + i * 64, coins, p);
-        shake_stream(coins, sraw + i * P.sraw, P.sraw, p);
+        uint32_t h[4];
+        sha3_256(h, ct + i * P.ct, P.ct, p);
+        store_words(hct + i * 32, h, p);
     }
+}
+
+// Decapsulation, after the decryption: m' from its bits, kr = G(m' | H(pk))
+// with H(pk) from the sk, secret stream from kr[32:64].
+__global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
+                                                       uint8_t *buf, uint8_t *kr, uint8_t *sraw, int64_t B)
+{
+    TMVP_PAIR(0);
+    if (t >= B) return;
+    uint32_t m[4];
+    // this lane's 32 bits of each message word: bits 64w + 32p + j
+    const uint32_t *mb = reinterpret_cast<const uint32_t *>(mbits + t * kN + 32 * p); // bit k in uint16 k
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint32_t two = mb[32 * w + j];
+            x |= ((two & 1u) << (2 * j)) | (((two >> 16) & 1u) << (2 * j + 1));
+        }
+        m[w] = x;
+    }
+    uint32_t h[4], coins[4];
+    load_words(h, sk + t * P.sk + P.sk - 64, p);
+    hash_g(m, h, buf + t * 64, kr + t * 64, coins, p);
+    shake_stream(coins, sraw + t * P.sraw, P.sraw, p);
 }
 
 // CBD_mu over the secret stream (S/src/poly.cu:836-916; SB/src/fips202.cu:625-651):
@@ -335,11 +347,12 @@ __global__ __launch_bounds__(64) void enc_hash2_kernel(KemParams P, const uint8_
     store_words(ss + t * 32, s, p);
 }
 
-// Decapsulation, last step: compare ct with the re-encryption, kr[32:] = H(ct),
-// replace kr[0:32] by z on mismatch, ss = H(kr).  fail[t] records the outcome.
+// Decapsulation, last step: compare ct with the re-encryption, kr[32:] = H(ct)
+// (computed beforehand), replace kr[0:32] by z on mismatch, ss = H(kr).
+// fail[t] records the outcome.
 __global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint8_t *ct, const uint8_t *ctp,
-                                                     const uint8_t *sk, uint8_t *kr, uint8_t *ss, uint8_t *fail,
-                                                     int64_t B)
+                                                     const uint8_t *sk, const uint8_t *hct, const uint8_t *kr,
+                                                     uint8_t *ss, uint8_t *fail, int64_t B)
 {
     TMVP_PAIR(0);
     if (t >= B) return;
@@ -350,7 +363,7 @@ __global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint8_t 
     diff |= partner(diff);
     const uint32_t mask = 0u - (uint32_t)(diff != 0); // all ones on mismatch, in both lanes
     uint32_t h[4], k[8], s[4], k0[4], z[4];
-    sha3_256(h, ct + t * P.ct, P.ct, p);
+    load_words(h, hct + t * 32, p);
     load_words(k0, kr + t * 64, p);
     load_words(z, sk + t * P.sk + P.sk - 32, p);
 #pragma unroll
@@ -526,12 +539,20 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
 }
 
 // Decapsulation; with `resident` set, the matrix A of every instance is
-// already in W.A and is not generated again.
+// already in W.A and is not generated again.  The matrix stream and H(ct)
+// depend only on the inputs and run on W.aux beside the decryption.
 static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *ct,
                                const uint8_t *sk, uint8_t *ss, uint8_t *fail, int64_t B, int *status, hipStream_t st,
                                bool resident)
 {
     const int L = P.l;
+    const int64_t t0 = resident ? B : 0;
+    TRY(hipEventRecord(W.fork, st));
+    TRY(hipStreamWaitEvent(W.aux, W.fork, 0));
+    hipLaunchKernelGGL(dec_pre_kernel, dim3(pair_blocks(2 * B - t0)), dim3(64), 0, W.aux, P, ct, sk, W.araw, W.hct, B,
+                       t0);
+    TRY(hipGetLastError());
+    TRY(hipEventRecord(W.join, W.aux));
     // IND-CPA decryption (S/src/cuda_kernel.cu:341-370; SB/src/SABLE_indcpa.cu:422-463)
     if (P.sable)
         TRY(unpack(sk, P.sk, 0, W.s, L * kN, 2, L * kN, kXfSk2, B, st));
@@ -541,14 +562,14 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     TRY(unpack(ct, P.ct, P.pvc, W.bits, kN, P.et, kN, kXfNone, B, st));
     TRY(product(P, backend, kInner, kEpiDec, W.poly, W.v2, W.s, W.bits, B, status, st));
     // re-encryption of m' under the pk stored in the sk
-    const int64_t t0 = resident ? B : 0;
-    hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(2 * B - t0)), dim3(64), 0, st, P, W.poly, sk, W.buf, W.kr,
-                       W.araw, W.sraw, B, t0);
+    hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.poly, sk, W.buf, W.kr, W.sraw, B);
     TRY(hipGetLastError());
-    if (!resident) TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
     TRY(gen_secret(P, W, B, st));
+    TRY(hipStreamWaitEvent(st, W.join, 0));
+    if (!resident) TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
     TRY(indcpa_enc_tail(P, W, backend, sk + P.indcpa_sk, P.sk, W.ctp, B, status, st));
-    hipLaunchKernelGGL(dec_fin_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.ctp, sk, W.kr, ss, fail, B);
+    hipLaunchKernelGGL(dec_fin_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.ctp, sk, W.hct, W.kr, ss, fail,
+                       B);
     return hipGetLastError();
 }
 
@@ -628,7 +649,8 @@ size_t kem_workspace_bytes(const KemParams &P, int64_t B)
     b += n * (size_t)P.sk;   // skg
     b += n * 32;             // hpkg
     b += n * (size_t)P.sraw; // sraw
-    return b + 16 * 256;
+    b += n * 32;             // hct
+    return b + 17 * 256;
 }
 
 void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorkspace &W)
@@ -653,6 +675,7 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
     W.skg = take(n * (size_t)P.sk);
     W.hpkg = take(n * 32);
     W.sraw = take(n * (size_t)P.sraw);
+    W.hct = take(n * 32);
 }
 
 } // namespace tmvp

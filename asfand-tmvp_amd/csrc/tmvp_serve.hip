@@ -70,6 +70,8 @@ bool is_pinned(const void *p)
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    hipStream_t aux = nullptr; // decapsulation side stream
+    hipEvent_t fork = nullptr, join = nullptr;
     uint8_t *ws = nullptr;    // KEM workspace
     uint8_t *d_in = nullptr;  // keys | inputs
     uint8_t *d_out = nullptr; // ct | ss | fail
@@ -130,7 +132,11 @@ void max_sizes(int64_t B, size_t &ws, size_t &in, size_t &out)
 void free_slot(Slot &s)
 {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.aux) (void)hipStreamSynchronize(s.aux);
     if (s.done) (void)hipEventDestroy(s.done);
+    if (s.fork) (void)hipEventDestroy(s.fork);
+    if (s.join) (void)hipEventDestroy(s.join);
+    if (s.aux) (void)hipStreamDestroy(s.aux);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     if (s.ws) (void)hipFree(s.ws);
     if (s.d_in) (void)hipFree(s.d_in);
@@ -146,6 +152,10 @@ int alloc_slot(tmvp_server *S, Slot &s)
 {
     if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return TMVP_EHIP;
     if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) return TMVP_EHIP;
+    if (hipStreamCreateWithFlags(&s.aux, hipStreamNonBlocking) != hipSuccess) return TMVP_EHIP;
+    if (hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess)
+        return TMVP_EHIP;
     if (hipMalloc(&s.ws, S->ws_bytes) != hipSuccess || hipMalloc(&s.d_in, S->in_bytes) != hipSuccess ||
         hipMalloc(&s.d_out, S->out_bytes) != hipSuccess || hipMalloc(&s.d_status, sizeof(int)) != hipSuccess)
         return TMVP_ENOMEM;
@@ -193,6 +203,9 @@ int launch_slot(tmvp_server *S, Slot &s)
         return TMVP_EHIP;
     KemWorkspace W;
     kem_workspace_carve(P, B, s.ws, W);
+    W.aux = s.aux;
+    W.fork = s.fork;
+    W.join = s.join;
     const Store &st = S->store[s.ps];
     KeyStore K{st.pk, st.sk, st.A, st.hpk};
     const int32_t *dkeys = reinterpret_cast<const int32_t *>(s.d_in);

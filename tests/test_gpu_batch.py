@@ -168,16 +168,17 @@ def test_graph_replay_matches_eager(ctx):
     jobs = config3_jobs(rng, 257)
     st = T.Stream()
     kb = 300
-    pk, _ = O.kem_keypair_batch("saber", rng.integers(0, 256, (kb, 96), dtype=np.uint8), nthreads=8)
+    pk, sk = O.kem_keypair_batch("saber", rng.integers(0, 256, (kb, 96), dtype=np.uint8), nthreads=8)
     rnd = rng.integers(0, 256, (kb, 32), dtype=np.uint8)
     want_ct, want_ss = O.kem_enc_batch("saber", rnd, pk, nthreads=8)
-    dpk, drnd = T.DeviceBuffer.from_numpy(pk), T.DeviceBuffer.from_numpy(rnd)
-    dct, dss = T.DeviceBuffer(kb * 1088), T.DeviceBuffer(kb * 32)
+    dpk, drnd, dsk = T.DeviceBuffer.from_numpy(pk), T.DeviceBuffer.from_numpy(rnd), T.DeviceBuffer.from_numpy(sk)
+    dct, dss, dss2, dfail = T.DeviceBuffer(kb * 1088), T.DeviceBuffer(kb * 32), T.DeviceBuffer(kb * 32), T.DeviceBuffer(kb)
     ctx.kem_reserve("saber", kb, stream=st)
 
-    def body():
+    def body():  # the decapsulation forks to its side stream and joins back inside the capture
         ctx.batch_launch([j.desc for j in jobs], stream=st)
         ctx.kem_encaps(dct, dss, dpk, drnd, kb, "saber", T.MFMA, stream=st)
+        ctx.kem_decaps(dss2, dct, dsk, kb, "saber", T.MFMA, fail=dfail, stream=st)
 
     g = ctx.capture(st, body)
     other = Job(rng, T.OP_MATVEC, "saber", 0, 99)
@@ -187,6 +188,8 @@ def test_graph_replay_matches_eager(ctx):
                 j.dout.zero()
             dct.zero()
             dss.zero()
+            dss2.zero()
+            dfail.zero()
             T.device_synchronize()
             g.launch(st)
             st.synchronize()
@@ -195,12 +198,14 @@ def test_graph_replay_matches_eager(ctx):
                 assert np.array_equal(j.got(), j.want), (rep, j.op, j.ps)
             assert np.array_equal(dct.download(np.uint8, (kb, -1)), want_ct), rep
             assert np.array_equal(dss.download(np.uint8, (kb, -1)), want_ss), rep
+            assert np.array_equal(dss2.download(np.uint8, (kb, -1)), want_ss), rep
+            assert not dfail.download(np.uint8).any(), rep
             run_list(ctx, [other])  # eager work between replays
             assert np.array_equal(other.got(), other.want)
     finally:
         g.destroy()
         st.destroy()
-    for b in (dpk, drnd, dct, dss):
+    for b in (dpk, drnd, dsk, dct, dss, dss2, dfail):
         b.free()
     for j in jobs + [other]:
         j.free()
