@@ -166,18 +166,23 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     // Registers holding one instance's inputs, term by term.
     u32x4 va[NT][ITJ];
     u32x2 vs[NT];
+    // Loads of term j of instance i.  Every load is issued on every path (an
+    // instance past the end reloads the current one, lanes without a chunk
+    // reload another chunk of the same term and ignore it), so the compiler
+    // sees one fixed sequence of memory operations per iteration and can wait
+    // for exactly the loads it needs; conditional loads make it wait for all.
     auto fetch_term = [&](int64_t i, int j) {
-        const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + i * (NPOLY * kN));
+        const int64_t ie = i < J.batch ? i : inst;
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + ie * (NPOLY * kN));
 #pragma unroll
         for (int it = 0; it < ITJ; it++) {
-            const int u = lane + 64 * it; // chunk of this term: row r = u >> 5, piece u & 31
-            if (CJ % 64 == 0 || u < CJ) {
-                const int r = u >> 5;
-                const int pq = (MODE == kMatVec) ? r * L + j : (MODE == kMatVecT) ? j * L + r : j;
-                va[j][it] = src[pq * 32 + (u & 31)];
-            }
+            const int u0 = lane + 64 * it;                             // chunk of this term: row u >> 5, piece u & 31
+            const int u = (CJ % 64 == 0 || u0 < CJ) ? u0 : u0 - 64;    // always a chunk of this term
+            const int r = u >> 5;
+            const int pq = (MODE == kMatVec) ? r * L + j : (MODE == kMatVecT) ? j * L + r : j;
+            va[j][it] = src[pq * 32 + (u & 31)];
         }
-        vs[j] = reinterpret_cast<const u32x2 *>(J.s + i * (NT * kN))[j * 64 + lane];
+        vs[j] = reinterpret_cast<const u32x2 *>(J.s + ie * (NT * kN))[j * 64 + lane];
     };
 #pragma unroll
     for (int j = 0; j < NT; j++) fetch_term(inst, j);
@@ -241,11 +246,20 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
 
+    // Every use of a term's input registers must come before the prefetch that
+    // reloads them: otherwise the compiler gives the load another register,
+    // copies it back at the loop edge and waits for it there (vmcnt), which
+    // exposes the full HBM latency once per instance.  The staging's LDS
+    // writes are ordered by the memory clobber; its one pure-register result,
+    // the range-check word, is routed through the barrier.
+    auto pin_after_stage = [&] { asm volatile("" : "+v"(bad) : : "memory"); };
+
     // Software pipeline: the staging of the following term (the next term of
     // this instance, or term 0 of the next instance) goes to the other slice
     // and is issued between the MFMAs of the current term.
     stage(0, 0);
-    if (inst + stride < J.batch) fetch_term(inst + stride, 0);
+    pin_after_stage();
+    fetch_term(inst + stride, 0);
     wave_fence();
     // Accumulators start at the rounding constant h in the lo-limb lanes, so
     // the epilogue never adds it.
@@ -282,10 +296,12 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
 
             if (j + 1 < NT) {
                 stage(j + 1, buf ^ 1);
-                if (next < J.batch) fetch_term(next, j + 1);
+                pin_after_stage();
+                fetch_term(next, j + 1);
             } else if (next < J.batch) {
                 stage(0, buf ^ 1); // va[0] / vs[0] already hold the next instance's term 0
-                if (next + stride < J.batch) fetch_term(next + stride, 0);
+                pin_after_stage();
+                fetch_term(next + stride, 0);
             }
 
             // K-chunk outermost: 2*NR independent accumulators between dependent MFMAs.
