@@ -336,8 +336,8 @@ Plan default_plan(uint64_t sig)
 }
 
 // Splits the resident workgroups between the jobs in proportion to
-// batch x products x weight; each job gets at least one workgroup and no
-// more than it has groups of four instances.
+// batch x per-instance cost x weight; each job gets at least one workgroup and
+// no more than it has groups of four instances.
 int build_multi(const tmvp_ctx *ctx, const WorkList &W, const Plan &P, MultiJob &M)
 {
     int per_cu = mfma_multi_per_cu();
@@ -346,7 +346,7 @@ int build_multi(const tmvp_ctx *ctx, const WorkList &W, const Plan &P, MultiJob 
     double total = 0.0;
     double work[TMVP_MAX_WORK];
     for (int i = 0; i < W.n; i++) {
-        work[i] = (double)W.J[i].batch * mfma_kind_products(W.kind[i]) * P.weight[W.kind[i]];
+        work[i] = (double)W.J[i].batch * mfma_kind_cost(W.kind[i]) * P.weight[W.kind[i]];
         total += work[i];
     }
     int begin = 0;

@@ -70,8 +70,8 @@ constexpr int kNumKinds = 22;
 constexpr int kMaxSegments = 16;
 // Kind of (mode, l, epi), or -1 when there is no such instantiation.
 int mfma_kind(int mode, int l, int epi);
-// Polynomial products per instance of a kind (rows x terms).
-int mfma_kind_products(int kind);
+// Relative time per instance of a kind (terms x (rows + 2)).
+int mfma_kind_cost(int kind);
 // A segment: workgroups [block_begin, block_begin + blocks) run job J.
 struct Segment {
     Job J;

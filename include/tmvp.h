@@ -140,10 +140,10 @@ int tmvp_innerprod(tmvp_ctx *ctx, int param_set, int backend, int epilogue, uint
  * block per instance (S/src/cuda_kernel.cu:129, :162; SB/src/SABLE_indcpa.cu:186,
  * :217).  A work list instead runs any mix of products, parameter sets and
  * epilogues in ONE launch of a persistent MFMA kernel: its workgroups are
- * split between the jobs in proportion to their products, with the split and
+ * split between the jobs in proportion to their work, with the split and
  * the workgroups per CU taken from a plan that tmvp_batch_autotune() measures
- * for the list's shape (kinds and batch sizes); without one, the split is by
- * product count at the kernel's full occupancy.  The VALU backends issue one
+ * for the list's shape (kinds and batch sizes); without one, the split follows
+ * a per-instance cost model (terms x (rows + 2)) at full occupancy.  The VALU backends issue one
  * launch per job.  Each descriptor has the meaning of the matching single
  * call: tmvp_polymul (eq_bits), tmvp_matvec (epilogue 0 = mod q, 1 = round to
  * p) or tmvp_innerprod (epilogue = tmvp_ip_epilogue). */
