@@ -5,9 +5,9 @@
 // A 64-bit rotation by r then needs the partner's half, fetched with one DPP
 // swap of adjacent lanes, and one v_alignbit: for 0 < r < 32 the new half is
 // alignbit(own, partner, 32 - r) in both lanes, for r > 32 it is
-// alignbit(partner, own, 64 - r).  A round costs about 160 VALU instructions
-// per lane, against about 290 for a whole state in one lane, and a batch gets
-// twice the waves.
+// alignbit(partner, own, 64 - r).  With three-input XORs and chi in v_bitop3,
+// a round costs about 125 VALU instructions per lane, against about 290 for a
+// whole state in one lane, and a batch gets twice the waves.
 //
 // Round constants come from the rc(t) LFSR of FIPS 202 Algorithm 5 and the
 // rho offsets from the (x, y) -> (y, 2x + 3y) walk of Algorithm 2, both
@@ -102,27 +102,37 @@ __device__ __forceinline__ void rho_pi_all(uint32_t (&B)[25], const uint32_t (&A
     (rho_pi<Is>(B, A), ...);
 }
 
+// a ^ b ^ c in one v_bitop3 (truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 __device__ __forceinline__ void f1600(uint32_t (&A)[25], int p)
 {
+    // The round constant is read at a uniform index (a scalar load) and the
+    // lane's half picked with a mask: a per-lane index would make it a vector
+    // load, and its wait would also wait for every store still in flight.
+    const uint32_t hi_mask = 0u - (uint32_t)p;
 #pragma unroll 1
     for (int round = 0; round < 24; round++) {
-        uint32_t C[5], Cp[5], B[25];
+        uint32_t C[5], R[5], B[25];
+        // theta: column parities C, R = rotl(C, 1), A ^= C[x-1] ^ R[x+1]
 #pragma unroll
-        for (int x = 0; x < 5; x++) C[x] = A[x] ^ A[x + 5] ^ A[x + 10] ^ A[x + 15] ^ A[x + 20];
+        for (int x = 0; x < 5; x++) C[x] = xor3(xor3(A[x], A[x + 5], A[x + 10]), A[x + 15], A[x + 20]);
 #pragma unroll
-        for (int x = 0; x < 5; x++) Cp[x] = partner(C[x]);
+        for (int x = 0; x < 5; x++) R[x] = rotl2<1>(C[x], partner(C[x]));
 #pragma unroll
-        for (int x = 0; x < 5; x++) {
-            const uint32_t D = C[(x + 4) % 5] ^ rotl2<1>(C[(x + 1) % 5], Cp[(x + 1) % 5]);
+        for (int x = 0; x < 5; x++)
 #pragma unroll
-            for (int y = 0; y < 25; y += 5) A[x + y] ^= D;
-        }
+            for (int y = 0; y < 25; y += 5) A[x + y] = xor3(A[x + y], C[(x + 4) % 5], R[(x + 1) % 5]);
         rho_pi_all(B, A, std::make_integer_sequence<int, 25>{});
 #pragma unroll
         for (int y = 0; y < 25; y += 5)
 #pragma unroll
             for (int x = 0; x < 5; x++) A[x + y] = B[x + y] ^ (~B[(x + 1) % 5 + y] & B[(x + 2) % 5 + y]);
-        A[0] ^= kRC2[2 * round + p];
+        const uint32_t rc_lo = kRC2[2 * round], rc_hi = kRC2[2 * round + 1];
+        A[0] ^= (rc_lo & ~hi_mask) | (rc_hi & hi_mask);
     }
 }
 
@@ -218,17 +228,23 @@ __device__ __forceinline__ void sha3_512_words(uint32_t (&out)[8], const uint32_
     for (int i = 0; i < 8; i++) out[i] = A[i];
 }
 
-// Squeeze `nwords` 64-bit words of output (absorbed, not yet permuted state)
-// to dst, each lane writing its halves.
-__device__ __forceinline__ void squeeze_words(uint32_t (&A)[25], uint8_t *dst, int nwords, int p)
+// Squeeze `nwords` 64-bit words of SHAKE-128 output (absorbed, not yet
+// permuted state), calling f(k, half) with this lane's half of word k.
+template <class F> __device__ __forceinline__ void squeeze_each(uint32_t (&A)[25], int nwords, int p, F f)
 {
-    uint32_t *d = reinterpret_cast<uint32_t *>(dst) + p;
     for (int k = 0; k < nwords; k += kShake128Rate / 8) {
         f1600(A, p);
 #pragma unroll
         for (int i = 0; i < kShake128Rate / 8; i++)
-            if (k + i < nwords) d[2 * (k + i)] = A[i];
+            if (k + i < nwords) f(k + i, A[i]);
     }
+}
+
+// Squeeze `nwords` words to dst, each lane writing its halves.
+__device__ __forceinline__ void squeeze_words(uint32_t (&A)[25], uint8_t *dst, int nwords, int p)
+{
+    uint32_t *d = reinterpret_cast<uint32_t *>(dst) + p;
+    squeeze_each(A, nwords, p, [&](int k, uint32_t h) { d[2 * k] = h; });
 }
 
 } // namespace keccak

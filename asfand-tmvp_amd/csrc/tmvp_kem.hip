@@ -47,6 +47,53 @@ __device__ __forceinline__ void shake_stream(const uint32_t (&seed)[4], uint8_t 
     squeeze_words(A, dst, bytes / 8, p);
 }
 
+// CBD_mu straight from the squeeze for mu = 8 and mu = 2, where every
+// coefficient lies inside one 32-bit half of a word: coefficient c is the
+// popcount of stream bits [c mu, c mu + mu/2) minus that of the next mu/2
+// bits (S/src/poly.cu:836-916, SB/src/fips202.cu:625-651), and bit i of this
+// lane's half of word k is stream bit 64k + 32p + i.  S is the instance's row
+// of L * 256 coefficients; qmask as in cbd_kernel.
+__device__ __forceinline__ bool secret_direct(const KemParams &P) { return P.mu == 8 || P.mu == 2; }
+
+__device__ __forceinline__ void cbd8_half(uint16_t *S, int k, uint32_t h, int p, uint32_t qmask)
+{
+    uint32_t x = h - ((h >> 1) & 0x55555555u);
+    x = (x & 0x33333333u) + ((x >> 2) & 0x33333333u); // popcount of every nibble
+    uint32_t e[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) e[j] = (((x >> (8 * j)) & 0xfu) - ((x >> (8 * j + 4)) & 0xfu)) & qmask;
+    *reinterpret_cast<uint2 *>(S + 8 * k + 4 * p) = make_uint2(e[0] | (e[1] << 16), e[2] | (e[3] << 16));
+}
+
+__device__ __forceinline__ void cbd2_half(uint16_t *S, int k, uint32_t h, int p, uint32_t qmask)
+{
+    const uint32_t a = h & 0x55555555u, b = (h >> 1) & 0x55555555u;
+    uint32_t o[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const uint32_t e0 = (((a >> (4 * q)) & 1u) - ((b >> (4 * q)) & 1u)) & qmask;
+        const uint32_t e1 = (((a >> (4 * q + 2)) & 1u) - ((b >> (4 * q + 2)) & 1u)) & qmask;
+        o[q] = e0 | (e1 << 16);
+    }
+    uint4 *d = reinterpret_cast<uint4 *>(S + 32 * k + 16 * p);
+    d[0] = make_uint4(o[0], o[1], o[2], o[3]);
+    d[1] = make_uint4(o[4], o[5], o[6], o[7]);
+}
+
+// The secret stream of one instance from its 32-byte seed: sampled into S
+// when secret_direct(P), else left raw in sraw for cbd_kernel.
+__device__ __forceinline__ void secret_stream(const KemParams &P, const uint32_t (&seed)[4], uint16_t *S,
+                                              uint8_t *sraw, int p)
+{
+    uint32_t A[25];
+    absorb_words(A, seed, kShake128Rate, kShakeDS, p);
+    const int nwords = P.sraw / 8;
+    const uint32_t qmask = P.sable ? (1u << P.eq) - 1u : 0xffffu;
+    if (P.mu == 8) squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd8_half(S, k, h, p, qmask); });
+    else if (P.mu == 2) squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd2_half(S, k, h, p, qmask); });
+    else squeeze_words(A, sraw, nwords, p);
+}
+
 // G(m | H(pk)) -> buf = m | H(pk), kr; returns the coins kr[32:64]; the
 // shared first step of encapsulation and of the re-encryption in
 // decapsulation (S/src/cuda_kernel.cu:116-118, :378-379).
@@ -66,15 +113,18 @@ __device__ __forceinline__ void hash_g(const uint32_t (&m)[4], const uint32_t (&
     for (int i = 0; i < 4; i++) coins[i] = k[4 + i];
 }
 
-// The front kernels run the long matrix sponge (pairs [0, B)) beside the rest
-// of the instance's hashing and its secret stream (pairs [B, 2B)); a kernel
-// starting at pair t0 = B skips the matrix.
+// The front kernels run the long matrix sponge (23 permutations for Saber;
+// pairs [0, B)) beside the rest of the instance's hashing and its secret
+// stream (pairs [B, 2B)), one wave per SIMD at B = 16384: a sponge wave alone
+// is latency-bound, and separate launches of the two roles share SIMDs and
+// slow the matrix sponge down.  A kernel starting at pair t0 = B skips the
+// matrix.
 
 // Key generation: seeds[t] = seed_A | seed_s | z.  seed_A' = SHAKE-128(seed_A)
-// goes to the pk tail and seeds the matrix stream; seed_s seeds the secret
-// stream; z goes to the sk tail.
+// goes to the pk tail (and the sk's copy of it) and seeds the matrix stream;
+// seed_s seeds the secret stream; z goes to the sk tail.
 __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
-                                                      uint8_t *araw, uint8_t *sraw, int64_t B)
+                                                      uint8_t *araw, uint16_t *S, uint8_t *sraw, int64_t B)
 {
     TMVP_PAIR(0);
     if (t >= 2 * B) return;
@@ -86,11 +136,12 @@ __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t
         f1600(st, p);
         const uint32_t sa[4] = {st[0], st[1], st[2], st[3]};
         store_words(pk + t * P.pk + P.pvc, sa, p);
+        store_words(sk + t * P.sk + P.indcpa_sk + P.pvc, sa, p);
         shake_stream(sa, araw + t * P.araw, P.araw, p);
     } else {
         const int64_t i = t - B;
         load_words(m, seeds + i * 96 + 32, p);
-        shake_stream(m, sraw + i * P.sraw, P.sraw, p);
+        secret_stream(P, m, S + i * P.l * kN, sraw + i * P.sraw, p);
         load_words(m, seeds + i * 96 + 64, p);
         store_words(sk + i * P.sk + P.sk - 32, m, p);
     }
@@ -100,7 +151,7 @@ __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t
 // secret stream from kr[32:64].  hpk (if not null) holds H(pk) per instance.
 __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
                                                        const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint8_t *araw,
-                                                       uint8_t *sraw, int64_t B, int64_t t0)
+                                                       uint16_t *S, uint8_t *sraw, int64_t B, int64_t t0)
 {
     TMVP_PAIR(t0);
     if (t >= 2 * B) return;
@@ -116,7 +167,7 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
         if (hpk) load_words(h, hpk + i * 32, p);
         else sha3_256(h, pk + i * P.pk, P.pk, p);
         hash_g(m, h, buf + i * 64, kr + i * 64, coins, p);
-        shake_stream(coins, sraw + i * P.sraw, P.sraw, p);
+        secret_stream(P, coins, S + i * P.l * kN, sraw + i * P.sraw, p);
     }
 }
 
@@ -143,7 +194,8 @@ __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t 
 // Decapsulation, after the decryption: m' from its bits, kr = G(m' | H(pk))
 // with H(pk) from the sk, secret stream from kr[32:64].
 __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
-                                                       uint8_t *buf, uint8_t *kr, uint8_t *sraw, int64_t B)
+                                                       uint8_t *buf, uint8_t *kr, uint16_t *S, uint8_t *sraw,
+                                                       int64_t B)
 {
     TMVP_PAIR(0);
     if (t >= B) return;
@@ -163,7 +215,7 @@ __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16
     uint32_t h[4], coins[4];
     load_words(h, sk + t * P.sk + P.sk - 64, p);
     hash_g(m, h, buf + t * 64, kr + t * 64, coins, p);
-    shake_stream(coins, sraw + t * P.sraw, P.sraw, p);
+    secret_stream(P, coins, S + t * P.l * kN, sraw + t * P.sraw, p);
 }
 
 // CBD_mu over the secret stream (S/src/poly.cu:836-916; SB/src/fips202.cu:625-651):
@@ -211,9 +263,12 @@ __global__ __launch_bounds__(256) void cbd_kernel(const uint8_t *sraw, int64_t s
 // unpacking (SB/src/pack.cu:366-377).
 enum Xf : int { kXfNone = 0, kXfSk2 = 1 };
 
+// With dst2 set, the packed words also go to dst2 + t * dst2_stride (+ the
+// same chunk offset).
 template <int W, int XF>
 __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t src_stride, uint8_t *dst,
-                                                   int64_t dst_stride, int dst_off, int chunks, int64_t B)
+                                                   int64_t dst_stride, int dst_off, int chunks, int64_t B,
+                                                   uint8_t *dst2, int64_t dst2_stride)
 {
     const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= B * chunks) return;
@@ -240,6 +295,11 @@ __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t 
     uint64_t *d = reinterpret_cast<uint64_t *>(dst + t * dst_stride + dst_off + (int64_t)c * W * 8);
 #pragma unroll
     for (int k = 0; k < W; k++) d[k] = words[k];
+    if (dst2) {
+        uint64_t *d2 = reinterpret_cast<uint64_t *>(dst2 + t * dst2_stride + dst_off + (int64_t)c * W * 8);
+#pragma unroll
+        for (int k = 0; k < W; k++) d2[k] = words[k];
+    }
 }
 
 template <int W, int XF>
@@ -272,19 +332,6 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *src, int64_t
     }
 }
 
-// Copy `len` bytes (multiple of 8) per instance.
-__global__ __launch_bounds__(256) void copy_kernel(const uint8_t *src, int64_t src_stride, int src_off, uint8_t *dst,
-                                                   int64_t dst_stride, int dst_off, int len, int64_t B)
-{
-    const int words = len / 8;
-    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= B * words) return;
-    const int64_t t = gid / words;
-    const int k = (int)(gid - t * words);
-    reinterpret_cast<uint64_t *>(dst + t * dst_stride + dst_off)[k] =
-        reinterpret_cast<const uint64_t *>(src + t * src_stride + src_off)[k];
-}
-
 // dst[t] = src[keys[t]] for `words` 8-byte words per row (key ids checked by the caller).
 __global__ __launch_bounds__(256) void gather_kernel(const uint8_t *src, int64_t src_stride, const int32_t *keys,
                                                      uint8_t *dst, int64_t dst_stride, int words, int64_t B)
@@ -309,13 +356,14 @@ __global__ __launch_bounds__(64) void hash_pk_kernel(KemParams P, const uint8_t 
     store_words(hpk + t * 32, h, p);
 }
 
-// The GenMatrix stream of each public key's seed (for a key store).
-__global__ __launch_bounds__(64) void matrix_stream_kernel(KemParams P, const uint8_t *pk, uint8_t *araw, int64_t B)
+// The GenMatrix stream from the 32-byte seed at seed + t * stride.
+__global__ __launch_bounds__(64) void matrix_stream_kernel(KemParams P, const uint8_t *seed, int64_t stride,
+                                                           uint8_t *araw, int64_t B)
 {
     TMVP_PAIR(0);
     if (t >= B) return;
     uint32_t m[4];
-    load_words(m, pk + t * P.pk + P.pvc, p);
+    load_words(m, seed + t * stride, p);
     shake_stream(m, araw + t * P.araw, P.araw, p);
 }
 
@@ -411,19 +459,20 @@ static hipError_t product(const KemParams &P, int backend, int mode, int epi, ui
 // Codec launch for a run-time width: finds the instantiation with W == w.
 template <int W>
 static hipError_t pack_w(const uint16_t *src, int64_t src_stride, uint8_t *dst, int64_t dst_stride, int dst_off, int w,
-                         int chunks, int xf, int64_t B, hipStream_t st)
+                         int chunks, int xf, int64_t B, hipStream_t st, uint8_t *dst2, int64_t dst2_stride)
 {
     if constexpr (W > 13) {
         return hipErrorInvalidValue;
     } else {
-        if (w != W) return pack_w<W + 1>(src, src_stride, dst, dst_stride, dst_off, w, chunks, xf, B, st);
+        if (w != W)
+            return pack_w<W + 1>(src, src_stride, dst, dst_stride, dst_off, w, chunks, xf, B, st, dst2, dst2_stride);
         const dim3 grid(blocks_for(B * chunks, 256));
         if (xf == kXfSk2)
             hipLaunchKernelGGL((pack_kernel<W, kXfSk2>), grid, dim3(256), 0, st, src, src_stride, dst, dst_stride,
-                               dst_off, chunks, B);
+                               dst_off, chunks, B, dst2, dst2_stride);
         else
             hipLaunchKernelGGL((pack_kernel<W, kXfNone>), grid, dim3(256), 0, st, src, src_stride, dst, dst_stride,
-                               dst_off, chunks, B);
+                               dst_off, chunks, B, dst2, dst2_stride);
         return hipGetLastError();
     }
 }
@@ -449,9 +498,10 @@ static hipError_t unpack_w(const uint8_t *src, int64_t src_stride, int src_off, 
 
 // ncoef is a multiple of 64; byte offsets and strides are multiples of 8.
 static hipError_t pack(const uint16_t *src, int64_t src_stride, uint8_t *dst, int64_t dst_stride, int dst_off, int w,
-                       int ncoef, int xf, int64_t B, hipStream_t st)
+                       int ncoef, int xf, int64_t B, hipStream_t st, uint8_t *dst2 = nullptr,
+                       int64_t dst2_stride = 0)
 {
-    return pack_w<1>(src, src_stride, dst, dst_stride, dst_off, w, ncoef / 64, xf, B, st);
+    return pack_w<1>(src, src_stride, dst, dst_stride, dst_off, w, ncoef / 64, xf, B, st, dst2, dst2_stride);
 }
 
 static hipError_t unpack(const uint8_t *src, int64_t src_stride, int src_off, uint16_t *dst, int64_t dst_stride,
@@ -471,12 +521,13 @@ static hipError_t cbd_launch(const KemParams &P, const KemWorkspace &W, int64_t 
     return hipGetLastError();
 }
 
+// After a front kernel: the secrets it did not sample itself (mu = 6, 10).
 static hipError_t gen_secret(const KemParams &P, const KemWorkspace &W, int64_t B, hipStream_t st)
 {
     switch (P.mu) {
-    case 2: return cbd_launch<2>(P, W, B, st);
+    case 2:
+    case 8: return hipSuccess; // sampled in the front kernel
     case 6: return cbd_launch<6>(P, W, B, st);
-    case 8: return cbd_launch<8>(P, W, B, st);
     case 10: return cbd_launch<10>(P, W, B, st);
     }
     return hipErrorInvalidValue;
@@ -485,15 +536,14 @@ static hipError_t gen_secret(const KemParams &P, const KemWorkspace &W, int64_t 
 // Threads for n lane pairs.
 static inline unsigned pair_blocks(int64_t n) { return blocks_for(2 * n, 64); }
 
-// IND-CPA encryption, after sampling: A and s' are in the workspace, the
-// message in buf[t][0:32] (S/src/cuda_kernel.cu:124-178).  ct gets
-// pack_EP(round(A s')) | pack_ET(round(b . s' - m)).
-static hipError_t indcpa_enc_tail(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *pk,
-                                  int64_t pk_stride, uint8_t *ct, int64_t B, int *status, hipStream_t st)
+// IND-CPA encryption (S/src/cuda_kernel.cu:124-178) in two halves.  The
+// InnerProd half needs only s' and the message (buf[t][0:32]) and writes
+// pack_ET(round(b . s' - m)) to the ct tail; the MatrixVectorMul half needs A
+// as well and writes pack_EP(round(A s')) to the ct head.
+static hipError_t enc_inner_half(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *pk,
+                                 int64_t pk_stride, uint8_t *ct, int64_t B, int *status, hipStream_t st)
 {
     const int L = P.l;
-    TRY(product(P, backend, kMatVec, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
-    TRY(pack(W.v1, L * kN, ct, P.ct, 0, P.ep, L * kN, kXfNone, B, st));
     TRY(unpack(pk, pk_stride, 0, W.v2, L * kN, P.ep, L * kN, kXfNone, B, st));
     TRY(unpack(W.buf, 64, 0, W.bits, kN, 1, kN, kXfNone, B, st)); // message bits (S/src/pack.cu:86-99)
     TRY(product(P, backend, kInner, kEpiEnc, W.poly, W.v2, W.s, W.bits, B, status, st));
@@ -501,25 +551,36 @@ static hipError_t indcpa_enc_tail(const KemParams &P, const KemWorkspace &W, int
     return hipSuccess;
 }
 
+static hipError_t enc_matvec_half(const KemParams &P, const KemWorkspace &W, int backend, uint8_t *ct, int64_t B,
+                                  int *status, hipStream_t st)
+{
+    const int L = P.l;
+    TRY(product(P, backend, kMatVec, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
+    TRY(pack(W.v1, L * kN, ct, P.ct, 0, P.ep, L * kN, kXfNone, B, st));
+    return hipSuccess;
+}
+
+// Key generation and encapsulation run on one stream: the front kernel keeps
+// one sponge wave per SIMD, and the steps after it are short and mostly
+// bandwidth-bound, so running them beside each other gains less than the
+// event waits cost.
 hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *seeds, uint8_t *pk,
                        uint8_t *sk, int64_t B, int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
     const int L = P.l;
-    hipLaunchKernelGGL(kp_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk, W.araw, W.sraw,
-                       B);
+    hipLaunchKernelGGL(kp_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk, W.araw, W.s,
+                       W.sraw, B);
     TRY(hipGetLastError());
     TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
     TRY(gen_secret(P, W, B, st));
     TRY(product(P, backend, kMatVecT, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
-    TRY(pack(W.v1, L * kN, pk, P.pk, 0, P.ep, L * kN, kXfNone, B, st));
+    // b to the pk and to the sk's copy of it
+    TRY(pack(W.v1, L * kN, pk, P.pk, 0, P.ep, L * kN, kXfNone, B, st, sk + P.indcpa_sk, P.sk));
     if (P.sable)
         TRY(pack(W.s, L * kN, sk, P.sk, 0, 2, L * kN, kXfSk2, B, st));
     else
         TRY(pack(W.s, L * kN, sk, P.sk, 0, P.eq, L * kN, kXfNone, B, st));
-    hipLaunchKernelGGL(copy_kernel, dim3(blocks_for(B * (P.pk / 8), 256)), dim3(256), 0, st, pk, (int64_t)P.pk, 0, sk,
-                       (int64_t)P.sk, P.indcpa_sk, P.pk, B);
-    TRY(hipGetLastError());
     hipLaunchKernelGGL(kp_hash_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, pk, sk, B);
     return hipGetLastError();
 }
@@ -529,18 +590,20 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
 {
     if (B <= 0) return hipSuccess;
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk, nullptr, W.buf, W.kr,
-                       W.araw, W.sraw, B, (int64_t)0);
+                       W.araw, W.s, W.sraw, B, (int64_t)0);
     TRY(hipGetLastError());
     TRY(unpack(W.araw, P.araw, 0, W.A, P.l * P.l * kN, P.eq, P.l * P.l * kN, kXfNone, B, st));
     TRY(gen_secret(P, W, B, st));
-    TRY(indcpa_enc_tail(P, W, backend, pk, P.pk, ct, B, status, st));
+    TRY(enc_matvec_half(P, W, backend, ct, B, status, st));
+    TRY(enc_inner_half(P, W, backend, pk, P.pk, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
 }
 
 // Decapsulation; with `resident` set, the matrix A of every instance is
-// already in W.A and is not generated again.  The matrix stream and H(ct)
-// depend only on the inputs and run on W.aux beside the decryption.
+// already in W.A and is not generated again.  The matrix stream (and its
+// unpacking) and H(ct) depend only on the inputs and run on W.aux beside the
+// decryption and the InnerProd half of the re-encryption.
 static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *ct,
                                const uint8_t *sk, uint8_t *ss, uint8_t *fail, int64_t B, int *status, hipStream_t st,
                                bool resident)
@@ -552,6 +615,7 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     hipLaunchKernelGGL(dec_pre_kernel, dim3(pair_blocks(2 * B - t0)), dim3(64), 0, W.aux, P, ct, sk, W.araw, W.hct, B,
                        t0);
     TRY(hipGetLastError());
+    if (!resident) TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, W.aux));
     TRY(hipEventRecord(W.join, W.aux));
     // IND-CPA decryption (S/src/cuda_kernel.cu:341-370; SB/src/SABLE_indcpa.cu:422-463)
     if (P.sable)
@@ -562,12 +626,13 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     TRY(unpack(ct, P.ct, P.pvc, W.bits, kN, P.et, kN, kXfNone, B, st));
     TRY(product(P, backend, kInner, kEpiDec, W.poly, W.v2, W.s, W.bits, B, status, st));
     // re-encryption of m' under the pk stored in the sk
-    hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.poly, sk, W.buf, W.kr, W.sraw, B);
+    hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.poly, sk, W.buf, W.kr, W.s, W.sraw,
+                       B);
     TRY(hipGetLastError());
     TRY(gen_secret(P, W, B, st));
+    TRY(enc_inner_half(P, W, backend, sk + P.indcpa_sk, P.sk, W.ctp, B, status, st));
     TRY(hipStreamWaitEvent(st, W.join, 0));
-    if (!resident) TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
-    TRY(indcpa_enc_tail(P, W, backend, sk + P.indcpa_sk, P.sk, W.ctp, B, status, st));
+    TRY(enc_matvec_half(P, W, backend, W.ctp, B, status, st));
     hipLaunchKernelGGL(dec_fin_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.ctp, sk, W.hct, W.kr, ss, fail,
                        B);
     return hipGetLastError();
@@ -600,10 +665,11 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
                B, st));
     TRY(gather(K.hpk, 32, keys, W.hpkg, 32, 32, B, st));
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg, W.buf, W.kr,
-                       W.araw, W.sraw, B, B);
+                       W.araw, W.s, W.sraw, B, B);
     TRY(hipGetLastError());
     TRY(gen_secret(P, W, B, st));
-    TRY(indcpa_enc_tail(P, W, backend, W.pkg, P.pk, ct, B, status, st));
+    TRY(enc_inner_half(P, W, backend, W.pkg, P.pk, ct, B, status, st));
+    TRY(enc_matvec_half(P, W, backend, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
 }
@@ -624,7 +690,8 @@ hipError_t kem_store_derive(const KemParams &P, const uint8_t *pk, int64_t n, ui
                             uint8_t *araw, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(matrix_stream_kernel, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk, araw, n);
+    hipLaunchKernelGGL(matrix_stream_kernel, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk + P.pvc, (int64_t)P.pk, araw,
+                       n);
     TRY(hipGetLastError());
     TRY(unpack(araw, P.araw, 0, A, P.l * P.l * kN, P.eq, P.l * P.l * kN, kXfNone, n, st));
     hipLaunchKernelGGL(hash_pk_kernel, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk, hpk, n);
