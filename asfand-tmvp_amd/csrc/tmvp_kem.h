@@ -14,14 +14,14 @@ struct KemParams {
     int pvc;       // L * EP * 32: compressed polynomial vector
     int indcpa_sk; // Saber: L * 13 * 32, Sable: L * 2 * 32
     int pk, sk, ct;
-    int araw; // L * L * EQ * 32: the SHAKE-128 stream GenMatrix unpacks
+    int araw; // L * L * EQ * 32: bytes of the SHAKE-128 stream GenMatrix unpacks
     int sraw; // L * MU * 32: the SHAKE-128 stream GenSecret samples from
 };
 
 // Scratch for one batched call (carved from one device allocation).
 struct KemWorkspace {
     uint16_t *A, *s, *v1, *v2, *bits, *poly;
-    uint8_t *buf, *kr, *ctp, *araw, *sraw;
+    uint8_t *buf, *kr, *ctp, *sraw;
     uint8_t *pkg, *skg, *hpkg; // key material gathered from a key store
     uint8_t *hct;              // H(ct) per instance (decapsulation)
     // Decapsulation runs the steps that depend only on its inputs (the matrix
@@ -60,8 +60,8 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
 hipError_t kem_decaps_keyed(const KemParams &P, const KemWorkspace &W, int backend, const KeyStore &K,
                             const int32_t *keys, const uint8_t *ct, uint8_t *ss, uint8_t *fail, int64_t B,
                             int *status, hipStream_t st);
-// A and H(pk) of n public keys; araw is scratch of n * P.araw bytes.
+// A and H(pk) of n public keys.
 hipError_t kem_store_derive(const KemParams &P, const uint8_t *pk, int64_t n, uint16_t *A, uint8_t *hpk,
-                            uint8_t *araw, hipStream_t st);
+                            hipStream_t st);
 
 } // namespace tmvp

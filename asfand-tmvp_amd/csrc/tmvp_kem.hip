@@ -36,16 +36,81 @@ using namespace keccak;
     const int64_t t = (((int64_t)blockIdx.x * 64 + threadIdx.x) >> 1) + (t0);                                       \
     const int p = (int)(threadIdx.x & 1)
 
-// The SHAKE-128(seed) stream of `bytes` bytes to dst: GenMatrix
-// (S/src/pack.cu:145-153, SB/src/pack.cu:41-50) and GenSecret
-// (S/src/poly.cu:1062-1073, SB/src/fips202.cu:655-668) unpack it afterwards
-// with kernels of one thread per 64 coefficients.
-__device__ __forceinline__ void shake_stream(const uint32_t (&seed)[4], uint8_t *dst, int bytes, int p)
+// GenMatrix (S/src/pack.cu:145-153 with the 13-bit unpacking of :51-65;
+// SB/src/pack.cu:22-50, 11 bits): the SHAKE-128(seed) stream read as EQ-bit
+// little-endian coefficients, in the order of A[i][j][k].  64 coefficients
+// are exactly EQ 64-bit words of the stream, so the squeezed words go through
+// a ring of three such groups in LDS (per instance; the pair's lanes write the
+// low and high dwords of each word) and every group the stream has completed
+// is unpacked at once, lane p taking coefficients 32p..32p+31 of it, which are
+// exactly its EQ dwords.  A block of 21 words leaves at most EQ - 1 words of
+// an unfinished group behind, so three groups always hold what is still
+// needed.  The ring's instance stride (78 dwords) keeps the 64 lanes' reads
+// and writes in distinct banks.
+constexpr int kRingDwords = 6 * 13; // per instance, for EQ <= 13
+
+template <int EQ>
+__device__ __forceinline__ void matrix_to_a(const uint32_t (&seed)[4], uint16_t *arow, int nwords, uint32_t *ring,
+                                            int p)
 {
-    uint32_t A[25];
-    absorb_words(A, seed, kShake128Rate, kShakeDS, p);
-    squeeze_words(A, dst, bytes / 8, p);
+    constexpr int RW = 3 * EQ; // ring words
+    constexpr int RB = kShake128Rate / 8;
+    static_assert(RB <= RW && 2 * RW <= kRingDwords, "ring size");
+    constexpr uint32_t mask = (1u << EQ) - 1u;
+    uint32_t st[25];
+    absorb_words(st, seed, kShake128Rate, kShakeDS, p);
+    int g = 0;  // next group to unpack
+    int kb = 0; // ring word of the block's first word
+    for (int k = 0; k < nwords; k += RB) {
+        f1600(st, p);
+#pragma unroll
+        for (int i = 0; i < RB; i++) {
+            if (k + i < nwords) {
+                int w = kb + i;
+                if (w >= RW) w -= RW;
+                ring[2 * w + p] = st[i];
+            }
+        }
+        kb += RB;
+        if (kb >= RW) kb -= RW;
+        // The pair's writes above are read by both lanes below; the wave's LDS
+        // operations execute in order, so only code motion must be stopped.
+        __builtin_amdgcn_wave_barrier();
+        const int avail = k + RB < nwords ? k + RB : nwords;
+        for (; EQ * (g + 1) <= avail; g++) {
+            const uint32_t *src = ring + 2 * EQ * (g % 3) + EQ * p;
+            uint32_t d[EQ];
+#pragma unroll
+            for (int i = 0; i < EQ; i++) d[i] = src[i];
+            uint32_t o[16];
+#pragma unroll
+            for (int j = 0; j < 32; j++) {
+                const int bit = EQ * j, i = bit >> 5, sh = bit & 31;
+                const uint32_t v =
+                    (sh + EQ <= 32 ? (d[i] >> sh) : __builtin_amdgcn_alignbit(d[i + 1 < EQ ? i + 1 : i], d[i], sh)) &
+                    mask;
+                if (j & 1) o[j >> 1] |= v << 16;
+                else o[j >> 1] = v;
+            }
+            uint4 *dst = reinterpret_cast<uint4 *>(arow + 64 * g + 32 * p);
+#pragma unroll
+            for (int q = 0; q < 4; q++) dst[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
 }
+
+// A[t] from the 32-byte seed; `ring` is the instance's kRingDwords in LDS.
+__device__ __forceinline__ void gen_matrix(const KemParams &P, const uint32_t (&seed)[4], uint16_t *arow,
+                                           uint32_t *ring, int p)
+{
+    if (P.eq == 13) matrix_to_a<13>(seed, arow, P.araw / 8, ring, p);
+    else matrix_to_a<11>(seed, arow, P.araw / 8, ring, p);
+}
+
+#define TMVP_RING                                                                                                    \
+    __shared__ uint32_t ring_lds[32 * kRingDwords];                                                                  \
+    uint32_t *const ring = ring_lds + (threadIdx.x >> 1) * kRingDwords
 
 // CBD_mu straight from the squeeze for mu = 8 and mu = 2, where every
 // coefficient lies inside one 32-bit half of a word: coefficient c is the
@@ -124,9 +189,10 @@ __device__ __forceinline__ void hash_g(const uint32_t (&m)[4], const uint32_t (&
 // goes to the pk tail (and the sk's copy of it) and seeds the matrix stream;
 // seed_s seeds the secret stream; z goes to the sk tail.
 __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
-                                                      uint8_t *araw, uint16_t *S, uint8_t *sraw, int64_t B)
+                                                      uint16_t *A, uint16_t *S, uint8_t *sraw, int64_t B)
 {
     TMVP_PAIR(0);
+    TMVP_RING;
     if (t >= 2 * B) return;
     uint32_t m[4];
     if (t < B) {
@@ -137,7 +203,7 @@ __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t
         const uint32_t sa[4] = {st[0], st[1], st[2], st[3]};
         store_words(pk + t * P.pk + P.pvc, sa, p);
         store_words(sk + t * P.sk + P.indcpa_sk + P.pvc, sa, p);
-        shake_stream(sa, araw + t * P.araw, P.araw, p);
+        gen_matrix(P, sa, A + t * P.l * P.l * kN, ring, p);
     } else {
         const int64_t i = t - B;
         load_words(m, seeds + i * 96 + 32, p);
@@ -150,15 +216,16 @@ __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t
 // Encapsulation: matrix stream from the pk tail; m = H(rnd), kr = G(m | H(pk)),
 // secret stream from kr[32:64].  hpk (if not null) holds H(pk) per instance.
 __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
-                                                       const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint8_t *araw,
+                                                       const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *A,
                                                        uint16_t *S, uint8_t *sraw, int64_t B, int64_t t0)
 {
     TMVP_PAIR(t0);
+    TMVP_RING;
     if (t >= 2 * B) return;
     uint32_t m[4];
     if (t < B) {
         load_words(m, pk + t * P.pk + P.pvc, p);
-        shake_stream(m, araw + t * P.araw, P.araw, p);
+        gen_matrix(P, m, A + t * P.l * P.l * kN, ring, p);
     } else {
         const int64_t i = t - B;
         uint32_t r[4], h[4], coins[4];
@@ -174,15 +241,16 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
 // Decapsulation, before the decryption and beside it (on the auxiliary
 // stream): the matrix stream from the pk inside the sk (pairs [t0, B); t0 = B
 // when A is resident) and H(ct) (pairs [B, 2B)).
-__global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t *ct, const uint8_t *sk, uint8_t *araw,
+__global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t *ct, const uint8_t *sk, uint16_t *A,
                                                      uint8_t *hct, int64_t B, int64_t t0)
 {
     TMVP_PAIR(t0);
+    TMVP_RING;
     if (t >= 2 * B) return;
     if (t < B) {
         uint32_t m[4];
         load_words(m, sk + t * P.sk + P.indcpa_sk + P.pvc, p);
-        shake_stream(m, araw + t * P.araw, P.araw, p);
+        gen_matrix(P, m, A + t * P.l * P.l * kN, ring, p);
     } else {
         const int64_t i = t - B;
         uint32_t h[4];
@@ -356,15 +424,16 @@ __global__ __launch_bounds__(64) void hash_pk_kernel(KemParams P, const uint8_t 
     store_words(hpk + t * 32, h, p);
 }
 
-// The GenMatrix stream from the 32-byte seed at seed + t * stride.
-__global__ __launch_bounds__(64) void matrix_stream_kernel(KemParams P, const uint8_t *seed, int64_t stride,
-                                                           uint8_t *araw, int64_t B)
+// GenMatrix from the 32-byte seed at seed + t * stride (for a key store).
+__global__ __launch_bounds__(64) void matrix_kernel(KemParams P, const uint8_t *seed, int64_t stride, uint16_t *A,
+                                                    int64_t B)
 {
     TMVP_PAIR(0);
+    TMVP_RING;
     if (t >= B) return;
     uint32_t m[4];
     load_words(m, seed + t * stride, p);
-    shake_stream(m, araw + t * P.araw, P.araw, p);
+    gen_matrix(P, m, A + t * P.l * P.l * kN, ring, p);
 }
 
 // Key generation, last step: H(pk) into the sk.
@@ -569,10 +638,9 @@ hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, c
 {
     if (B <= 0) return hipSuccess;
     const int L = P.l;
-    hipLaunchKernelGGL(kp_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk, W.araw, W.s,
-                       W.sraw, B);
+    hipLaunchKernelGGL(kp_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk, W.A, W.s, W.sraw,
+                       B);
     TRY(hipGetLastError());
-    TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, st));
     TRY(gen_secret(P, W, B, st));
     TRY(product(P, backend, kMatVecT, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
     // b to the pk and to the sk's copy of it
@@ -590,9 +658,8 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
 {
     if (B <= 0) return hipSuccess;
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk, nullptr, W.buf, W.kr,
-                       W.araw, W.s, W.sraw, B, (int64_t)0);
+                       W.A, W.s, W.sraw, B, (int64_t)0);
     TRY(hipGetLastError());
-    TRY(unpack(W.araw, P.araw, 0, W.A, P.l * P.l * kN, P.eq, P.l * P.l * kN, kXfNone, B, st));
     TRY(gen_secret(P, W, B, st));
     TRY(enc_matvec_half(P, W, backend, ct, B, status, st));
     TRY(enc_inner_half(P, W, backend, pk, P.pk, ct, B, status, st));
@@ -612,10 +679,8 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     const int64_t t0 = resident ? B : 0;
     TRY(hipEventRecord(W.fork, st));
     TRY(hipStreamWaitEvent(W.aux, W.fork, 0));
-    hipLaunchKernelGGL(dec_pre_kernel, dim3(pair_blocks(2 * B - t0)), dim3(64), 0, W.aux, P, ct, sk, W.araw, W.hct, B,
-                       t0);
+    hipLaunchKernelGGL(dec_pre_kernel, dim3(pair_blocks(2 * B - t0)), dim3(64), 0, W.aux, P, ct, sk, W.A, W.hct, B, t0);
     TRY(hipGetLastError());
-    if (!resident) TRY(unpack(W.araw, P.araw, 0, W.A, L * L * kN, P.eq, L * L * kN, kXfNone, B, W.aux));
     TRY(hipEventRecord(W.join, W.aux));
     // IND-CPA decryption (S/src/cuda_kernel.cu:341-370; SB/src/SABLE_indcpa.cu:422-463)
     if (P.sable)
@@ -665,7 +730,7 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
                B, st));
     TRY(gather(K.hpk, 32, keys, W.hpkg, 32, 32, B, st));
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg, W.buf, W.kr,
-                       W.araw, W.s, W.sraw, B, B);
+                       W.A, W.s, W.sraw, B, B);
     TRY(hipGetLastError());
     TRY(gen_secret(P, W, B, st));
     TRY(enc_inner_half(P, W, backend, W.pkg, P.pk, ct, B, status, st));
@@ -687,13 +752,11 @@ hipError_t kem_decaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
 }
 
 hipError_t kem_store_derive(const KemParams &P, const uint8_t *pk, int64_t n, uint16_t *A, uint8_t *hpk,
-                            uint8_t *araw, hipStream_t st)
+                            hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(matrix_stream_kernel, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk + P.pvc, (int64_t)P.pk, araw,
-                       n);
+    hipLaunchKernelGGL(matrix_kernel, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk + P.pvc, (int64_t)P.pk, A, n);
     TRY(hipGetLastError());
-    TRY(unpack(araw, P.araw, 0, A, P.l * P.l * kN, P.eq, P.l * P.l * kN, kXfNone, n, st));
     hipLaunchKernelGGL(hash_pk_kernel, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk, hpk, n);
     return hipGetLastError();
 }
@@ -711,13 +774,12 @@ size_t kem_workspace_bytes(const KemParams &P, int64_t B)
     b += n * 64;             // buf
     b += n * 64;             // kr
     b += n * (size_t)P.ct;   // ctp
-    b += n * (size_t)P.araw; // araw
     b += n * (size_t)P.pk;   // pkg
     b += n * (size_t)P.sk;   // skg
     b += n * 32;             // hpkg
     b += n * (size_t)P.sraw; // sraw
     b += n * 32;             // hct
-    return b + 17 * 256;
+    return b + 16 * 256;
 }
 
 void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorkspace &W)
@@ -737,7 +799,6 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
     W.buf = take(n * 64);
     W.kr = take(n * 64);
     W.ctp = take(n * (size_t)P.ct);
-    W.araw = take(n * (size_t)P.araw);
     W.pkg = take(n * (size_t)P.pk);
     W.skg = take(n * (size_t)P.sk);
     W.hpkg = take(n * 32);

@@ -499,18 +499,15 @@ int tmvp_srv_add_keys(tmvp_server *S, int ps, const uint8_t *h_pk, const uint8_t
     if (st.n == 0) st.has_sk = h_sk != nullptr;
     int rc = grow_store(st, P, st.n + n, st.has_sk);
     if (rc) return rc;
-    uint8_t *araw = nullptr;
-    if (hipMalloc(&araw, (size_t)n * P.araw) != hipSuccess) return TMVP_ENOMEM;
     const size_t abytes = (size_t)P.l * P.l * kN * 2;
     rc = TMVP_OK;
     if (hipMemcpy(st.pk + (size_t)st.n * P.pk, h_pk, (size_t)n * P.pk, hipMemcpyHostToDevice) != hipSuccess ||
         (h_sk && hipMemcpy(st.sk + (size_t)st.n * P.sk, h_sk, (size_t)n * P.sk, hipMemcpyHostToDevice) != hipSuccess))
         rc = TMVP_EHIP;
     if (!rc && kem_store_derive(P, st.pk + (size_t)st.n * P.pk, n, st.A + (size_t)st.n * (abytes / 2),
-                                st.hpk + (size_t)st.n * 32, araw, nullptr) != hipSuccess)
+                                st.hpk + (size_t)st.n * 32, nullptr) != hipSuccess)
         rc = TMVP_EHIP;
     if (!rc && hipDeviceSynchronize() != hipSuccess) rc = TMVP_EHIP;
-    (void)hipFree(araw);
     if (!rc) st.n += n;
     return rc;
 }
