@@ -49,6 +49,66 @@ using namespace keccak;
 // and writes in distinct banks.
 constexpr int kRingDwords = 6 * 13; // per instance, for EQ <= 13
 
+// The 32 W-bit coefficients packed LSB-first in d[0..W) to dst (16-byte aligned).
+template <int W> __device__ __forceinline__ void unpack32(const uint32_t (&d)[W], uint16_t *dst)
+{
+    constexpr uint32_t mask = (1u << W) - 1u;
+    uint32_t o[16];
+#pragma unroll
+    for (int j = 0; j < 32; j++) {
+        const int bit = W * j, i = bit >> 5, sh = bit & 31;
+        const uint32_t v =
+            (sh + W <= 32 ? (d[i] >> sh) : __builtin_amdgcn_alignbit(d[i + 1 < W ? i + 1 : i], d[i], sh)) & mask;
+        if (j & 1) o[j >> 1] |= v << 16;
+        else o[j >> 1] = v;
+    }
+    uint4 *q4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+    for (int q = 0; q < 4; q++) q4[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+}
+
+// A packed W-bit vector of ncoef coefficients (ncoef a multiple of 256) at
+// src, unpacked to dst by a lane pair: 64 coefficients are W 64-bit words, and
+// lane p takes coefficients 32p..32p+31 of each group, its W dwords.  Four
+// groups' loads are issued before their stores (stores in flight hold up the
+// next loads' wait).
+template <int W> __device__ __forceinline__ void unpack_pair(const uint8_t *src, uint16_t *dst, int ncoef, int p)
+{
+    for (int g0 = 0; g0 < ncoef / 64; g0 += 4) {
+        uint32_t d[4][W];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t *s = reinterpret_cast<const uint32_t *>(src + 8 * W * (g0 + u)) + W * p;
+#pragma unroll
+            for (int i = 0; i < W; i++) d[u][i] = s[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) unpack32<W>(d[u], dst + 64 * (g0 + u) + 32 * p);
+    }
+}
+
+// The vector b of a public key (its first pvc bytes, EP bits per coefficient).
+__device__ __forceinline__ void unpack_b(const KemParams &P, const uint8_t *pk, uint16_t *dst, int p)
+{
+    if (P.ep == 10) unpack_pair<10>(pk, dst, P.l * kN, p);
+    else unpack_pair<9>(pk, dst, P.l * kN, p);
+}
+
+// The message bits as one 0/1 coefficient each (bit k of m is coefficient k):
+// this lane's halves of the four words cover coefficients 64w + 32p + j.
+__device__ __forceinline__ void message_bits(const uint32_t (&m)[4], uint16_t *dst, int p)
+{
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        uint32_t o[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) o[q] = ((m[w] >> (2 * q)) & 1u) | (((m[w] >> (2 * q + 1)) & 1u) << 16);
+        uint4 *q4 = reinterpret_cast<uint4 *>(dst + 64 * w + 32 * p);
+#pragma unroll
+        for (int q = 0; q < 4; q++) q4[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    }
+}
+
 template <int EQ>
 __device__ __forceinline__ void matrix_to_a(const uint32_t (&seed)[4], uint16_t *arow, int nwords, uint32_t *ring,
                                             int p)
@@ -56,7 +116,6 @@ __device__ __forceinline__ void matrix_to_a(const uint32_t (&seed)[4], uint16_t 
     constexpr int RW = 3 * EQ; // ring words
     constexpr int RB = kShake128Rate / 8;
     static_assert(RB <= RW && 2 * RW <= kRingDwords, "ring size");
-    constexpr uint32_t mask = (1u << EQ) - 1u;
     uint32_t st[25];
     absorb_words(st, seed, kShake128Rate, kShakeDS, p);
     int g = 0;  // next group to unpack
@@ -82,19 +141,7 @@ __device__ __forceinline__ void matrix_to_a(const uint32_t (&seed)[4], uint16_t 
             uint32_t d[EQ];
 #pragma unroll
             for (int i = 0; i < EQ; i++) d[i] = src[i];
-            uint32_t o[16];
-#pragma unroll
-            for (int j = 0; j < 32; j++) {
-                const int bit = EQ * j, i = bit >> 5, sh = bit & 31;
-                const uint32_t v =
-                    (sh + EQ <= 32 ? (d[i] >> sh) : __builtin_amdgcn_alignbit(d[i + 1 < EQ ? i + 1 : i], d[i], sh)) &
-                    mask;
-                if (j & 1) o[j >> 1] |= v << 16;
-                else o[j >> 1] = v;
-            }
-            uint4 *dst = reinterpret_cast<uint4 *>(arow + 64 * g + 32 * p);
-#pragma unroll
-            for (int q = 0; q < 4; q++) dst[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+            unpack32<EQ>(d, arow + 64 * g + 32 * p);
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -213,11 +260,13 @@ __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t
     }
 }
 
-// Encapsulation: matrix stream from the pk tail; m = H(rnd), kr = G(m | H(pk)),
-// secret stream from kr[32:64].  hpk (if not null) holds H(pk) per instance.
+// Encapsulation: matrix from the pk tail; m = H(rnd), kr = G(m | H(pk)),
+// secret from kr[32:64], and the InnerProd operands: b of the pk to V and m's
+// bits to bits.  hpk (if not null) holds H(pk) per instance.
 __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
                                                        const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *A,
-                                                       uint16_t *S, uint8_t *sraw, int64_t B, int64_t t0)
+                                                       uint16_t *S, uint8_t *sraw, uint16_t *V, uint16_t *bits,
+                                                       int64_t B, int64_t t0)
 {
     TMVP_PAIR(t0);
     TMVP_RING;
@@ -235,6 +284,8 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
         else sha3_256(h, pk + i * P.pk, P.pk, p);
         hash_g(m, h, buf + i * 64, kr + i * 64, coins, p);
         secret_stream(P, coins, S + i * P.l * kN, sraw + i * P.sraw, p);
+        message_bits(m, bits + i * kN, p);
+        unpack_b(P, pk + i * P.pk, V + i * P.l * kN, p);
     }
 }
 
@@ -260,10 +311,11 @@ __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t 
 }
 
 // Decapsulation, after the decryption: m' from its bits, kr = G(m' | H(pk))
-// with H(pk) from the sk, secret stream from kr[32:64].
+// with H(pk) from the sk, secret from kr[32:64], and b of the pk inside the
+// sk to V for the re-encryption.
 __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
                                                        uint8_t *buf, uint8_t *kr, uint16_t *S, uint8_t *sraw,
-                                                       int64_t B)
+                                                       uint16_t *V, int64_t B)
 {
     TMVP_PAIR(0);
     if (t >= B) return;
@@ -284,6 +336,7 @@ __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16
     load_words(h, sk + t * P.sk + P.sk - 64, p);
     hash_g(m, h, buf + t * 64, kr + t * 64, coins, p);
     secret_stream(P, coins, S + t * P.l * kN, sraw + t * P.sraw, p);
+    unpack_b(P, sk + t * P.sk + P.indcpa_sk, V + t * P.l * kN, p);
 }
 
 // CBD_mu over the secret stream (S/src/poly.cu:836-916; SB/src/fips202.cu:625-651):
@@ -606,15 +659,13 @@ static hipError_t gen_secret(const KemParams &P, const KemWorkspace &W, int64_t 
 static inline unsigned pair_blocks(int64_t n) { return blocks_for(2 * n, 64); }
 
 // IND-CPA encryption (S/src/cuda_kernel.cu:124-178) in two halves.  The
-// InnerProd half needs only s' and the message (buf[t][0:32]) and writes
-// pack_ET(round(b . s' - m)) to the ct tail; the MatrixVectorMul half needs A
-// as well and writes pack_EP(round(A s')) to the ct head.
-static hipError_t enc_inner_half(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *pk,
-                                 int64_t pk_stride, uint8_t *ct, int64_t B, int *status, hipStream_t st)
+// InnerProd half needs s', b of the pk (unpacked in W.v2) and the message's
+// bits (W.bits, one per coefficient: S/src/pack.cu:86-99), all written by the
+// front kernels, and writes pack_ET(round(b . s' - m)) to the ct tail; the
+// MatrixVectorMul half needs A and writes pack_EP(round(A s')) to the ct head.
+static hipError_t enc_inner_half(const KemParams &P, const KemWorkspace &W, int backend, uint8_t *ct, int64_t B,
+                                 int *status, hipStream_t st)
 {
-    const int L = P.l;
-    TRY(unpack(pk, pk_stride, 0, W.v2, L * kN, P.ep, L * kN, kXfNone, B, st));
-    TRY(unpack(W.buf, 64, 0, W.bits, kN, 1, kN, kXfNone, B, st)); // message bits (S/src/pack.cu:86-99)
     TRY(product(P, backend, kInner, kEpiEnc, W.poly, W.v2, W.s, W.bits, B, status, st));
     TRY(pack(W.poly, kN, ct, P.ct, P.pvc, P.et, kN, kXfNone, B, st));
     return hipSuccess;
@@ -658,11 +709,11 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
 {
     if (B <= 0) return hipSuccess;
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk, nullptr, W.buf, W.kr,
-                       W.A, W.s, W.sraw, B, (int64_t)0);
+                       W.A, W.s, W.sraw, W.v2, W.bits, B, (int64_t)0);
     TRY(hipGetLastError());
     TRY(gen_secret(P, W, B, st));
     TRY(enc_matvec_half(P, W, backend, ct, B, status, st));
-    TRY(enc_inner_half(P, W, backend, pk, P.pk, ct, B, status, st));
+    TRY(enc_inner_half(P, W, backend, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
 }
@@ -688,14 +739,15 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     else
         TRY(unpack(sk, P.sk, 0, W.s, L * kN, P.eq, L * kN, kXfNone, B, st));
     TRY(unpack(ct, P.ct, 0, W.v2, L * kN, P.ep, L * kN, kXfNone, B, st));
-    TRY(unpack(ct, P.ct, P.pvc, W.bits, kN, P.et, kN, kXfNone, B, st));
-    TRY(product(P, backend, kInner, kEpiDec, W.poly, W.v2, W.s, W.bits, B, status, st));
+    TRY(unpack(ct, P.ct, P.pvc, W.poly, kN, P.et, kN, kXfNone, B, st));
+    // m' bits to W.bits, where the re-encryption's InnerProd reads them
+    TRY(product(P, backend, kInner, kEpiDec, W.bits, W.v2, W.s, W.poly, B, status, st));
     // re-encryption of m' under the pk stored in the sk
-    hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.poly, sk, W.buf, W.kr, W.s, W.sraw,
-                       B);
+    hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.bits, sk, W.buf, W.kr, W.s, W.sraw,
+                       W.v2, B);
     TRY(hipGetLastError());
     TRY(gen_secret(P, W, B, st));
-    TRY(enc_inner_half(P, W, backend, sk + P.indcpa_sk, P.sk, W.ctp, B, status, st));
+    TRY(enc_inner_half(P, W, backend, W.ctp, B, status, st));
     TRY(hipStreamWaitEvent(st, W.join, 0));
     TRY(enc_matvec_half(P, W, backend, W.ctp, B, status, st));
     hipLaunchKernelGGL(dec_fin_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.ctp, sk, W.hct, W.kr, ss, fail,
@@ -730,10 +782,10 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
                B, st));
     TRY(gather(K.hpk, 32, keys, W.hpkg, 32, 32, B, st));
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg, W.buf, W.kr,
-                       W.A, W.s, W.sraw, B, B);
+                       W.A, W.s, W.sraw, W.v2, W.bits, B, B);
     TRY(hipGetLastError());
     TRY(gen_secret(P, W, B, st));
-    TRY(enc_inner_half(P, W, backend, W.pkg, P.pk, ct, B, status, st));
+    TRY(enc_inner_half(P, W, backend, ct, B, status, st));
     TRY(enc_matvec_half(P, W, backend, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
