@@ -529,7 +529,18 @@ __global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint8_t 
     const uint32_t *a = reinterpret_cast<const uint32_t *>(ct + t * P.ct) + p;
     const uint32_t *b = reinterpret_cast<const uint32_t *>(ctp + t * P.ct) + p;
     uint32_t diff = 0;
-    for (int i = 0; i < P.ct / 8; i++) diff |= a[2 * i] ^ b[2 * i];
+    int i = 0;
+    for (; i + 8 <= P.ct / 8; i += 8) { // eight words' loads in flight at a time
+        uint32_t x[8], y[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            x[u] = a[2 * (i + u)];
+            y[u] = b[2 * (i + u)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) diff |= x[u] ^ y[u];
+    }
+    for (; i < P.ct / 8; i++) diff |= a[2 * i] ^ b[2 * i];
     diff |= partner(diff);
     const uint32_t mask = 0u - (uint32_t)(diff != 0); // all ones on mismatch, in both lanes
     uint32_t h[4], k[8], s[4], k0[4], z[4];
