@@ -31,12 +31,15 @@
 // aligned 16-byte LDS read per MFMA pair (it feeds both halves h).
 //
 // An A-operand lane needs 16 consecutive band bytes at an offset whose
-// misalignment (i + 1) mod 8 is fixed per lane.  The band is therefore stored
-// eight times, shifted by 0..7 bytes, and every lane reads its own copy with
-// one 8-byte-aligned two-address LDS load (unaligned 16-byte LDS loads are
-// exact on gfx950 but many times slower).  K indices run in reversed jj order inside each
-// 16-byte group (the E blocks are stored byte-reversed) so that these windows
-// ascend in memory.
+// misalignment (i + 1) mod 4 is fixed per lane.  The band is therefore stored
+// four times, shifted by 0..3 bytes, and every lane reads a dword-aligned
+// window of its own copy (unaligned LDS loads are exact on gfx950 but many
+// times slower).  K indices run in reversed jj order inside each 16-byte group
+// (the E blocks are stored byte-reversed) so that these windows ascend in
+// memory.  A two-dword LDS read is served one dword per lane at a time, and a
+// pass is conflict-free when the 64 lanes hit 64 distinct banks: each copy's
+// lanes read 16 consecutive dwords, and the copies sit at 320 c (+ 4 for c > 0)
+// bytes so that the four windows tile the 64 banks exactly.
 //
 // Launch: persistent, 256 threads = 4 waves per workgroup.  A wave owns one
 // instance at a time in a private LDS slice, stages it one secret term at a
@@ -67,6 +70,10 @@ __device__ __forceinline__ uint32_t add16x2(uint32_t a, uint32_t b)
 constexpr int kEBlock = 32;           // bytes per E block (32 coefficients, one limb)
 constexpr int kEPoly = 16 * kEBlock;  // blocks e = 0..15 (e = 0 unused)
 constexpr int kBand = 288;            // bytes of one band copy: X[x] = ts(x - 32), x < 288
+// Byte offset of band copy c (see the bank note above); the four copies take kBandBytes.
+__host__ __device__ constexpr int band_copy(int c) { return 320 * c + (c ? 4 : 0); }
+constexpr int kBandBytes = 1264; // band_copy(3) + kBand, rounded up to 16
+static_assert(band_copy(3) + kBand <= kBandBytes && kBandBytes % 16 == 0, "band copies");
 
 // Bytewise negation of four bytes that are each <= 0x80.
 __device__ __forceinline__ uint32_t neg_small(uint32_t v) { return (0x80808080u - v) ^ 0x80808080u; }
@@ -117,7 +124,7 @@ template <int L, int MODE> struct Slice {
     static constexpr int CJ = NR * 32;          // 8-coefficient chunks of one term
     static constexpr int ITJ = (CJ + 63) / 64;  // staging loads per lane per term
     static constexpr int E_BYTES = NR * 2 * kEPoly;
-    static constexpr int BYTES = E_BYTES + 4 * kBand;
+    static constexpr int BYTES = E_BYTES + kBandBytes;
 };
 
 // DPP wave_shl:1 - lane i receives lane i + 1 (lane 63 receives 0).
@@ -140,7 +147,9 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
 {
     using S = Slice<L, MODE>;
     constexpr int NR = S::NR, NT = S::NT, NPOLY = S::NPOLY, CJ = S::CJ, ITJ = S::ITJ;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // The wave index is uniform; readfirstlane lets the compiler keep the
+    // instance loop (addresses, bounds, branches) in scalar registers.
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     int64_t inst = first;
     if (inst >= J.batch) return; // wave-uniform: each wave owns private LDS slices, no barriers
     uint8_t *const slice0 = lds + wave * 2 * S::BYTES;
@@ -159,7 +168,7 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     // A operand: dword index of the window start x0 - sh in band copy `sh`, without
     // the compile-time part 16p + 4h (x0 = 32D + 16h - 16(g&1) + il + 17, D = 2p + (g>>1);
     // il + 17 - sh is a multiple of 4).
-    const int abase = (sh * kBand) / 4 + 8 * (g >> 1) - 4 * (g & 1) + ((il + 17 - sh) >> 2);
+    const int abase = band_copy(sh) / 4 + 8 * (g >> 1) - 4 * (g & 1) + ((il + 17 - sh) >> 2);
     // B operand: byte offset of E[R - (g>>1) + 8] (minus 64 p per K-chunk) in the lane's limb plane
     const int bbase = limb * kEPoly + 16 * (1 - (g & 1)) + (R - (g >> 1) + 8) * kEBlock;
 
@@ -207,10 +216,9 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
             uint8_t *slot = E + r * 2 * kEPoly + bpos;
             *(uint2 *)(slot + (blk + 8) * kEBlock) = plo;
             *(uint2 *)(slot + kEPoly + (blk + 8) * kEBlock) = phi;
-            if (blk > 0) {
-                *(uint2 *)(slot + blk * kEBlock) = make_uint2(neg_small(plo.x), neg_small(plo.y));
-                *(uint2 *)(slot + kEPoly + blk * kEBlock) = make_uint2(neg_small(phi.x), neg_small(phi.y));
-            }
+            // negated copy; for blk = 0 it lands in block 0, which no MFMA reads
+            *(uint2 *)(slot + blk * kEBlock) = make_uint2(neg_small(plo.x), neg_small(plo.y));
+            *(uint2 *)(slot + kEPoly + blk * kEBlock) = make_uint2(neg_small(phi.x), neg_small(phi.y));
         }
         // ---- secret s_j -> int8 band X[x] = ts(x - 32), stored 4x shifted: copy c
         //      holds X[x + c] at byte x.  For w >= 8 the centred int8 is the low byte.
@@ -228,12 +236,12 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
         uint8_t *band = X + 32 + 4 * lane;
         *(uint32_t *)band = pos;
 #pragma unroll
-        for (int c = 1; c < 4; c++) *(uint32_t *)(band + c * kBand) = __builtin_amdgcn_alignbyte(p1, pos, c);
+        for (int c = 1; c < 4; c++) *(uint32_t *)(band + band_copy(c)) = __builtin_amdgcn_alignbyte(p1, pos, c);
         if (lane >= 56) {
             uint8_t *nb = X + 4 * lane - 224;
             *(uint32_t *)nb = neg;
 #pragma unroll
-            for (int c = 1; c < 4; c++) *(uint32_t *)(nb + c * kBand) = __builtin_amdgcn_alignbyte(n1, neg, c);
+            for (int c = 1; c < 4; c++) *(uint32_t *)(nb + band_copy(c)) = __builtin_amdgcn_alignbyte(n1, neg, c);
         }
     };
     // The slices are private to this wave and the LDS completes one wave's
@@ -373,7 +381,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEU
     // Two slices per wave: term t is staged into slice t & 1 while the term
     // before it is read from the other one.
     __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsBytes<L, MODE>];
-    mfma_body<L, MODE, EPI>(J, (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6), (int64_t)gridDim.x * 4, lds);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    mfma_body<L, MODE, EPI>(J, (int64_t)blockIdx.x * 4 + wave, (int64_t)gridDim.x * 4, lds);
 }
 
 // Persistent launch: enough workgroups to fill every CU at the occupancy the
@@ -471,7 +480,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
     int k = 0;
     while (k + 1 < M.nseg && (int)blockIdx.x >= M.seg[k + 1].block_begin) k++;
     const Segment &G = M.seg[k];
-    const int64_t first = (int64_t)((int)blockIdx.x - G.block_begin) * 4 + (threadIdx.x >> 6);
+    const int64_t first =
+        (int64_t)((int)blockIdx.x - G.block_begin) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t stride = (int64_t)G.blocks * 4;
     switch (G.kind) {
 #define TMVP_KIND(K)                                                                                                 \
