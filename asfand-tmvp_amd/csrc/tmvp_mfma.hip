@@ -289,14 +289,17 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
             int aoff = buf * S::BYTES + S::E_BYTES + 4 * abase;
             asm volatile("" : "+v"(aoff));
             const uint32_t *aw = reinterpret_cast<const uint32_t *>(slice0 + aoff);
-            v4i afr[4][2];
-#pragma unroll
-            for (int p = 0; p < 4; p++)
+            // A fragments of one K-chunk (both halves); chunk p + 1 is read while
+            // chunk p is multiplied.
+            auto read_a = [&](int p, v4i (&f)[2]) {
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const uint32_t *w = aw + 16 * p + 4 * h;
-                    afr[p][h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+                    f[h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
                 }
+            };
+            v4i acur[2];
+            read_a(0, acur);
             // B operand of K-chunk 0; chunk p + 1 is read while chunk p is multiplied.
             v4i bcur[NR];
 #pragma unroll
@@ -315,28 +318,31 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
             // K-chunk outermost: 2*NR independent accumulators between dependent MFMAs.
 #pragma unroll
             for (int p = 0; p < 4; p++) {
-                v4i bnxt[NR];
+                v4i bnxt[NR], anxt[2];
                 if (p < 3) {
+                    read_a(p + 1, anxt);
 #pragma unroll
                     for (int r = 0; r < NR; r++) bnxt[r] = *(const v4i *)(E + r * 2 * kEPoly + bbase - 64 * (p + 1));
                 }
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
                     const bool first = (j == 0 && p == 0);
-                    acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][0], bcur[r], first ? init : acc[r][0], 0, 0, 0);
-                    acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(afr[p][1], bcur[r], first ? init : acc[r][1], 0, 0, 0);
+                    acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur[0], bcur[r], first ? init : acc[r][0], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur[1], bcur[r], first ? init : acc[r][1], 0, 0, 0);
                 }
                 if (p < 3) {
+                    acur[0] = anxt[0];
+                    acur[1] = anxt[1];
 #pragma unroll
                     for (int r = 0; r < NR; r++) bcur[r] = bnxt[r];
                 }
             }
-            // Schedule: fragment reads for chunk 0 first, then per chunk the next
-            // chunk's B reads and one MFMA per three staging VALU instructions.
-            __builtin_amdgcn_sched_group_barrier(0x100, 16 + NR, 0);
+            // Schedule: chunk 0's fragment reads first, then per chunk the next
+            // chunk's A and B reads and one MFMA per three staging VALU instructions.
+            __builtin_amdgcn_sched_group_barrier(0x100, 4 + NR, 0);
 #pragma unroll
             for (int p = 0; p < 4; p++) {
-                if (p < 3) __builtin_amdgcn_sched_group_barrier(0x100, NR, 0);
+                if (p < 3) __builtin_amdgcn_sched_group_barrier(0x100, 4 + NR, 0);
 #pragma unroll
                 for (int k = 0; k < 2 * NR; k++) {
                     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
