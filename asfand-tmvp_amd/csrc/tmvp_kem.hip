@@ -13,7 +13,10 @@
 // SHAKE-128, b = round(A^T s)).
 //
 // Sponges run on a pair of lanes per instance (state in registers, split in
-// 32-bit halves); codecs and CBD sampling run one thread per 64 coefficients.
+// 32-bit halves) and write their results in the form the products read: A
+// (through an LDS ring), s (CBD for mu = 8 and 2), b of the public key and the
+// message bits.  The other codecs, and CBD for mu = 6 and 10, run one thread
+// per 64 coefficients.
 #include <string.h>
 
 #include <utility>
