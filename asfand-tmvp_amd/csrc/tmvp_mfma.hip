@@ -36,10 +36,11 @@
 // window of its own copy (unaligned LDS loads are exact on gfx950 but many
 // times slower).  K indices run in reversed jj order inside each 16-byte group
 // (the E blocks are stored byte-reversed) so that these windows ascend in
-// memory.  A two-dword LDS read is served one dword per lane at a time, and a
-// pass is conflict-free when the 64 lanes hit 64 distinct banks: each copy's
-// lanes read 16 consecutive dwords, and the copies sit at 320 c (+ 4 for c > 0)
-// bytes so that the four windows tile the 64 banks exactly.
+// memory.  The LDS serves a two-dword read one dword and 32 lanes at a time
+// from 32 banks: in each half-wave, each copy's 8 lanes read 8 consecutive
+// dwords, and the copies sit at 288 c (+ 4 for c > 0) bytes so that the four
+// 8-dword windows fall in distinct banks (at a plain 288-byte stride every
+// pass has a two-way conflict).
 //
 // Launch: persistent, 256 threads = 4 waves per workgroup.  A wave owns one
 // instance at a time in a private LDS slice, stages it one secret term at a
@@ -71,8 +72,8 @@ constexpr int kEBlock = 32;           // bytes per E block (32 coefficients, one
 constexpr int kEPoly = 16 * kEBlock;  // blocks e = 0..15 (e = 0 unused)
 constexpr int kBand = 288;            // bytes of one band copy: X[x] = ts(x - 32), x < 288
 // Byte offset of band copy c (see the bank note above); the four copies take kBandBytes.
-__host__ __device__ constexpr int band_copy(int c) { return 320 * c + (c ? 4 : 0); }
-constexpr int kBandBytes = 1264; // band_copy(3) + kBand, rounded up to 16
+__host__ __device__ constexpr int band_copy(int c) { return 288 * c + (c ? 4 : 0); }
+constexpr int kBandBytes = 1168; // band_copy(3) + kBand, rounded up to 16
 static_assert(band_copy(3) + kBand <= kBandBytes && kBandBytes % 16 == 0, "band copies");
 
 // Bytewise negation of four bytes that are each <= 0x80.
