@@ -15,13 +15,13 @@ struct KemParams {
     int indcpa_sk; // Saber: L * 13 * 32, Sable: L * 2 * 32
     int pk, sk, ct;
     int araw; // L * L * EQ * 32: bytes of the SHAKE-128 stream GenMatrix unpacks
-    int sraw; // L * MU * 32: the SHAKE-128 stream GenSecret samples from
+    int sraw; // L * MU * 32: bytes of the SHAKE-128 stream GenSecret samples from
 };
 
 // Scratch for one batched call (carved from one device allocation).
 struct KemWorkspace {
     uint16_t *A, *s, *v1, *v2, *bits, *poly;
-    uint8_t *buf, *kr, *ctp, *sraw;
+    uint8_t *buf, *kr, *ctp;
     uint8_t *pkg, *skg, *hpkg; // key material gathered from a key store
     uint8_t *hct;              // H(ct) per instance (decapsulation)
     // Decapsulation runs the steps that depend only on its inputs (the matrix

@@ -112,16 +112,23 @@ __device__ __forceinline__ void message_bits(const uint32_t (&m)[4], uint16_t *d
     }
 }
 
-template <int EQ>
-__device__ __forceinline__ void matrix_to_a(const uint32_t (&seed)[4], uint16_t *arow, int nwords, uint32_t *ring,
-                                            int p)
+// The SHAKE-128(seed) stream of nwords words, handed to f(g, d) one group of
+// G words at a time: d holds this lane's G dwords of group g (dwords G p ..
+// G p + G - 1 of the group's 2G).  The squeezed words go through a ring of RG
+// groups in LDS (the instance's kRingDwords; the pair's lanes write the low
+// and high dwords of each word), and every group the stream has completed is
+// handed over at once.  A block of 21 words leaves at most G - 1 words of an
+// unfinished group behind, so RG groups of at least 21 + G - 1 words always
+// hold what is still needed.
+template <int G, int RG, class F>
+__device__ __forceinline__ void stream_groups(const uint32_t (&seed)[4], int nwords, uint32_t *ring, int p, F f)
 {
-    constexpr int RW = 3 * EQ; // ring words
+    constexpr int RW = RG * G; // ring words
     constexpr int RB = kShake128Rate / 8;
-    static_assert(RB <= RW && 2 * RW <= kRingDwords, "ring size");
+    static_assert(RW >= RB + G - 1 && 2 * RW <= kRingDwords, "ring size");
     uint32_t st[25];
     absorb_words(st, seed, kShake128Rate, kShakeDS, p);
-    int g = 0;  // next group to unpack
+    int g = 0;  // next group to hand over
     int kb = 0; // ring word of the block's first word
     for (int k = 0; k < nwords; k += RB) {
         f1600(st, p);
@@ -139,15 +146,24 @@ __device__ __forceinline__ void matrix_to_a(const uint32_t (&seed)[4], uint16_t 
         // operations execute in order, so only code motion must be stopped.
         __builtin_amdgcn_wave_barrier();
         const int avail = k + RB < nwords ? k + RB : nwords;
-        for (; EQ * (g + 1) <= avail; g++) {
-            const uint32_t *src = ring + 2 * EQ * (g % 3) + EQ * p;
-            uint32_t d[EQ];
+        for (; G * (g + 1) <= avail; g++) {
+            const uint32_t *src = ring + 2 * G * (g % RG) + G * p;
+            uint32_t d[G];
 #pragma unroll
-            for (int i = 0; i < EQ; i++) d[i] = src[i];
-            unpack32<EQ>(d, arow + 64 * g + 32 * p);
+            for (int i = 0; i < G; i++) d[i] = src[i];
+            f(g, d);
         }
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// GenMatrix: 64 coefficients of EQ bits are exactly EQ words of the stream.
+template <int EQ>
+__device__ __forceinline__ void matrix_to_a(const uint32_t (&seed)[4], uint16_t *arow, int nwords, uint32_t *ring,
+                                            int p)
+{
+    stream_groups<EQ, 3>(seed, nwords, ring, p,
+                         [&](int g, const uint32_t(&d)[EQ]) { unpack32<EQ>(d, arow + 64 * g + 32 * p); });
 }
 
 // A[t] from the 32-byte seed; `ring` is the instance's kRingDwords in LDS.
@@ -162,13 +178,14 @@ __device__ __forceinline__ void gen_matrix(const KemParams &P, const uint32_t (&
     __shared__ uint32_t ring_lds[32 * kRingDwords];                                                                  \
     uint32_t *const ring = ring_lds + (threadIdx.x >> 1) * kRingDwords
 
-// CBD_mu straight from the squeeze for mu = 8 and mu = 2, where every
-// coefficient lies inside one 32-bit half of a word: coefficient c is the
-// popcount of stream bits [c mu, c mu + mu/2) minus that of the next mu/2
-// bits (S/src/poly.cu:836-916, SB/src/fips202.cu:625-651), and bit i of this
-// lane's half of word k is stream bit 64k + 32p + i.  S is the instance's row
-// of L * 256 coefficients; qmask as in cbd_kernel.
-__device__ __forceinline__ bool secret_direct(const KemParams &P) { return P.mu == 8 || P.mu == 2; }
+// CBD_mu (S/src/poly.cu:836-916, SB/src/fips202.cu:625-651): coefficient c
+// is the popcount of stream bits [c mu, c mu + mu/2) minus that of the next
+// mu/2 bits, stored as int16 (Saber) or masked mod q (Sable) by qmask.  For
+// mu = 8 and mu = 2 every coefficient lies inside one 32-bit half of a word
+// (bit i of this lane's half of word k is stream bit 64k + 32p + i) and is
+// sampled straight from the squeeze; for mu = 6 and 10 the coefficients
+// straddle the halves, and 64 of them are exactly mu words, so they go through
+// the LDS ring.  S is the instance's row of L * 256 coefficients.
 
 __device__ __forceinline__ void cbd8_half(uint16_t *S, int k, uint32_t h, int p, uint32_t qmask)
 {
@@ -195,18 +212,43 @@ __device__ __forceinline__ void cbd2_half(uint16_t *S, int k, uint32_t h, int p,
     d[1] = make_uint4(o[4], o[5], o[6], o[7]);
 }
 
-// The secret stream of one instance from its 32-byte seed: sampled into S
-// when secret_direct(P), else left raw in sraw for cbd_kernel.
-__device__ __forceinline__ void secret_stream(const KemParams &P, const uint32_t (&seed)[4], uint16_t *S,
-                                              uint8_t *sraw, int p)
+// The 32 coefficients whose MU-bit fields are packed LSB-first in d[0..MU).
+template <int MU> __device__ __forceinline__ void cbd32(const uint32_t (&d)[MU], uint16_t *dst, uint32_t qmask)
 {
-    uint32_t A[25];
-    absorb_words(A, seed, kShake128Rate, kShakeDS, p);
+    constexpr int H = MU / 2;
+    constexpr uint32_t lo = (1u << H) - 1u;
+    uint32_t o[16];
+#pragma unroll
+    for (int j = 0; j < 32; j++) {
+        const int bit = MU * j, i = bit >> 5, sh = bit & 31;
+        const uint32_t x = sh + MU <= 32 ? (d[i] >> sh) : __builtin_amdgcn_alignbit(d[i + 1 < MU ? i + 1 : i], d[i], sh);
+        const uint32_t v = ((uint32_t)__builtin_popcount(x & lo) - (uint32_t)__builtin_popcount((x >> H) & lo)) & qmask;
+        if (j & 1) o[j >> 1] |= v << 16;
+        else o[j >> 1] = v;
+    }
+    uint4 *q4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+    for (int q = 0; q < 4; q++) q4[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+}
+
+// The secret of one instance from its 32-byte seed, sampled into S.
+__device__ __forceinline__ void secret_stream(const KemParams &P, const uint32_t (&seed)[4], uint16_t *S,
+                                              uint32_t *ring, int p)
+{
     const int nwords = P.sraw / 8;
     const uint32_t qmask = P.sable ? (1u << P.eq) - 1u : 0xffffu;
-    if (P.mu == 8) squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd8_half(S, k, h, p, qmask); });
-    else if (P.mu == 2) squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd2_half(S, k, h, p, qmask); });
-    else squeeze_words(A, sraw, nwords, p);
+    if (P.mu == 8 || P.mu == 2) {
+        uint32_t A[25];
+        absorb_words(A, seed, kShake128Rate, kShakeDS, p);
+        if (P.mu == 8) squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd8_half(S, k, h, p, qmask); });
+        else squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd2_half(S, k, h, p, qmask); });
+    } else if (P.mu == 6) {
+        stream_groups<6, 5>(seed, nwords, ring, p,
+                            [&](int g, const uint32_t(&d)[6]) { cbd32<6>(d, S + 64 * g + 32 * p, qmask); });
+    } else {
+        stream_groups<10, 3>(seed, nwords, ring, p,
+                             [&](int g, const uint32_t(&d)[10]) { cbd32<10>(d, S + 64 * g + 32 * p, qmask); });
+    }
 }
 
 // G(m | H(pk)) -> buf = m | H(pk), kr; returns the coins kr[32:64]; the
@@ -239,7 +281,7 @@ __device__ __forceinline__ void hash_g(const uint32_t (&m)[4], const uint32_t (&
 // goes to the pk tail (and the sk's copy of it) and seeds the matrix stream;
 // seed_s seeds the secret stream; z goes to the sk tail.
 __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
-                                                      uint16_t *A, uint16_t *S, uint8_t *sraw, int64_t B)
+                                                      uint16_t *A, uint16_t *S, int64_t B)
 {
     TMVP_PAIR(0);
     TMVP_RING;
@@ -257,7 +299,7 @@ __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t
     } else {
         const int64_t i = t - B;
         load_words(m, seeds + i * 96 + 32, p);
-        secret_stream(P, m, S + i * P.l * kN, sraw + i * P.sraw, p);
+        secret_stream(P, m, S + i * P.l * kN, ring, p);
         load_words(m, seeds + i * 96 + 64, p);
         store_words(sk + i * P.sk + P.sk - 32, m, p);
     }
@@ -268,7 +310,7 @@ __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t
 // bits to bits.  hpk (if not null) holds H(pk) per instance.
 __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
                                                        const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *A,
-                                                       uint16_t *S, uint8_t *sraw, uint16_t *V, uint16_t *bits,
+                                                       uint16_t *S, uint16_t *V, uint16_t *bits,
                                                        int64_t B, int64_t t0)
 {
     TMVP_PAIR(t0);
@@ -286,7 +328,7 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
         if (hpk) load_words(h, hpk + i * 32, p);
         else sha3_256(h, pk + i * P.pk, P.pk, p);
         hash_g(m, h, buf + i * 64, kr + i * 64, coins, p);
-        secret_stream(P, coins, S + i * P.l * kN, sraw + i * P.sraw, p);
+        secret_stream(P, coins, S + i * P.l * kN, ring, p);
         message_bits(m, bits + i * kN, p);
         unpack_b(P, pk + i * P.pk, V + i * P.l * kN, p);
     }
@@ -317,10 +359,11 @@ __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t 
 // with H(pk) from the sk, secret from kr[32:64], and b of the pk inside the
 // sk to V for the re-encryption.
 __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
-                                                       uint8_t *buf, uint8_t *kr, uint16_t *S, uint8_t *sraw,
-                                                       uint16_t *V, int64_t B)
+                                                       uint8_t *buf, uint8_t *kr, uint16_t *S, uint16_t *V,
+                                                       int64_t B)
 {
     TMVP_PAIR(0);
+    TMVP_RING;
     if (t >= B) return;
     uint32_t m[4];
     // this lane's 32 bits of each message word: bits 64w + 32p + j
@@ -338,43 +381,8 @@ __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16
     uint32_t h[4], coins[4];
     load_words(h, sk + t * P.sk + P.sk - 64, p);
     hash_g(m, h, buf + t * 64, kr + t * 64, coins, p);
-    secret_stream(P, coins, S + t * P.l * kN, sraw + t * P.sraw, p);
+    secret_stream(P, coins, S + t * P.l * kN, ring, p);
     unpack_b(P, sk + t * P.sk + P.indcpa_sk, V + t * P.l * kN, p);
-}
-
-// CBD_mu over the secret stream (S/src/poly.cu:836-916; SB/src/fips202.cu:625-651):
-// coefficient k is the popcount of bits [k*mu, k*mu + mu/2) minus that of the
-// next mu/2 bits; one thread per 64 coefficients, which is exactly mu words.
-// Saber stores the value as int16, Sable masked mod q.
-template <int MU>
-__global__ __launch_bounds__(256) void cbd_kernel(const uint8_t *sraw, int64_t sraw_stride, uint16_t *S,
-                                                  int64_t s_stride, int chunks, uint32_t qmask, int64_t B)
-{
-    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= B * chunks) return;
-    const int64_t t = gid / chunks;
-    const int c = (int)(gid - t * chunks);
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(sraw + t * sraw_stride + (int64_t)c * MU * 8);
-    uint64_t words[MU];
-#pragma unroll
-    for (int k = 0; k < MU; k++) words[k] = src[k];
-    constexpr int H = MU / 2;
-    constexpr uint32_t lo = (1u << H) - 1u;
-    uint4 *d = reinterpret_cast<uint4 *>(S + t * s_stride + c * 64);
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        uint32_t e[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int bit = (8 * q + j) * MU, wi = bit >> 6, o = bit & 63;
-            uint64_t x = words[wi] >> o;
-            if (o + MU > 64) x |= words[wi + 1] << (64 - o);
-            const uint32_t bits = (uint32_t)x;
-            const int v = __builtin_popcount(bits & lo) - __builtin_popcount((bits >> H) & lo);
-            e[j] = (uint32_t)v & qmask;
-        }
-        d[q] = make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16), e[6] | (e[7] << 16));
-    }
 }
 
 // ---------------------------------------------------------------- codecs
@@ -646,29 +654,6 @@ static hipError_t unpack(const uint8_t *src, int64_t src_stride, int src_off, ui
     return unpack_w<1>(src, src_stride, src_off, dst, dst_stride, w, ncoef / 64, xf, B, st);
 }
 
-// CBD_mu over the secret streams of B instances into W.s.
-template <int MU>
-static hipError_t cbd_launch(const KemParams &P, const KemWorkspace &W, int64_t B, hipStream_t st)
-{
-    const int chunks = P.l * kN / 64;
-    const uint32_t qmask = P.sable ? (1u << P.eq) - 1u : 0xffffu;
-    hipLaunchKernelGGL((cbd_kernel<MU>), dim3(blocks_for(B * chunks, 256)), dim3(256), 0, st, W.sraw,
-                       (int64_t)P.sraw, W.s, (int64_t)(P.l * kN), chunks, qmask, B);
-    return hipGetLastError();
-}
-
-// After a front kernel: the secrets it did not sample itself (mu = 6, 10).
-static hipError_t gen_secret(const KemParams &P, const KemWorkspace &W, int64_t B, hipStream_t st)
-{
-    switch (P.mu) {
-    case 2:
-    case 8: return hipSuccess; // sampled in the front kernel
-    case 6: return cbd_launch<6>(P, W, B, st);
-    case 10: return cbd_launch<10>(P, W, B, st);
-    }
-    return hipErrorInvalidValue;
-}
-
 // Threads for n lane pairs.
 static inline unsigned pair_blocks(int64_t n) { return blocks_for(2 * n, 64); }
 
@@ -703,10 +688,8 @@ hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, c
 {
     if (B <= 0) return hipSuccess;
     const int L = P.l;
-    hipLaunchKernelGGL(kp_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk, W.A, W.s, W.sraw,
-                       B);
+    hipLaunchKernelGGL(kp_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk, W.A, W.s, B);
     TRY(hipGetLastError());
-    TRY(gen_secret(P, W, B, st));
     TRY(product(P, backend, kMatVecT, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
     // b to the pk and to the sk's copy of it
     TRY(pack(W.v1, L * kN, pk, P.pk, 0, P.ep, L * kN, kXfNone, B, st, sk + P.indcpa_sk, P.sk));
@@ -723,9 +706,8 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
 {
     if (B <= 0) return hipSuccess;
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk, nullptr, W.buf, W.kr,
-                       W.A, W.s, W.sraw, W.v2, W.bits, B, (int64_t)0);
+                       W.A, W.s, W.v2, W.bits, B, (int64_t)0);
     TRY(hipGetLastError());
-    TRY(gen_secret(P, W, B, st));
     TRY(enc_matvec_half(P, W, backend, ct, B, status, st));
     TRY(enc_inner_half(P, W, backend, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
@@ -757,10 +739,9 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     // m' bits to W.bits, where the re-encryption's InnerProd reads them
     TRY(product(P, backend, kInner, kEpiDec, W.bits, W.v2, W.s, W.poly, B, status, st));
     // re-encryption of m' under the pk stored in the sk
-    hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.bits, sk, W.buf, W.kr, W.s, W.sraw,
-                       W.v2, B);
+    hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.bits, sk, W.buf, W.kr, W.s, W.v2,
+                       B);
     TRY(hipGetLastError());
-    TRY(gen_secret(P, W, B, st));
     TRY(enc_inner_half(P, W, backend, W.ctp, B, status, st));
     TRY(hipStreamWaitEvent(st, W.join, 0));
     TRY(enc_matvec_half(P, W, backend, W.ctp, B, status, st));
@@ -796,9 +777,8 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
                B, st));
     TRY(gather(K.hpk, 32, keys, W.hpkg, 32, 32, B, st));
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg, W.buf, W.kr,
-                       W.A, W.s, W.sraw, W.v2, W.bits, B, B);
+                       W.A, W.s, W.v2, W.bits, B, B);
     TRY(hipGetLastError());
-    TRY(gen_secret(P, W, B, st));
     TRY(enc_inner_half(P, W, backend, ct, B, status, st));
     TRY(enc_matvec_half(P, W, backend, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
@@ -843,9 +823,8 @@ size_t kem_workspace_bytes(const KemParams &P, int64_t B)
     b += n * (size_t)P.pk;   // pkg
     b += n * (size_t)P.sk;   // skg
     b += n * 32;             // hpkg
-    b += n * (size_t)P.sraw; // sraw
     b += n * 32;             // hct
-    return b + 16 * 256;
+    return b + 15 * 256;
 }
 
 void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorkspace &W)
@@ -868,7 +847,6 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
     W.pkg = take(n * (size_t)P.pk);
     W.skg = take(n * (size_t)P.sk);
     W.hpkg = take(n * 32);
-    W.sraw = take(n * (size_t)P.sraw);
     W.hct = take(n * 32);
 }
 
