@@ -277,37 +277,69 @@ __device__ __forceinline__ void hash_g(const uint32_t (&m)[4], const uint32_t (&
 // slow the matrix sponge down.  A kernel starting at pair t0 = B skips the
 // matrix.
 
-// Key generation: seeds[t] = seed_A | seed_s | z.  seed_A' = SHAKE-128(seed_A)
-// goes to the pk tail (and the sk's copy of it) and seeds the matrix stream;
-// seed_s seeds the secret stream; z goes to the sk tail.
+// Key generation: seeds[t] = seed_A | seed_s | z.  Matrix role: seed_A' =
+// SHAKE-128(seed_A) goes to the pk tail (and the sk's copy of it) and seeds
+// GenMatrix.  Secret role: seed_s seeds the secret; z goes to the sk tail.
+__device__ __forceinline__ void kp_matrix_role(const KemParams &P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
+                                               uint16_t *A, int64_t t, uint32_t *ring, int p)
+{
+    uint32_t m[4], st[25];
+    load_words(m, seeds + t * 96, p);
+    absorb_words(st, m, kShake128Rate, kShakeDS, p);
+    f1600(st, p);
+    const uint32_t sa[4] = {st[0], st[1], st[2], st[3]};
+    store_words(pk + t * P.pk + P.pvc, sa, p);
+    store_words(sk + t * P.sk + P.indcpa_sk + P.pvc, sa, p);
+    gen_matrix(P, sa, A + t * P.l * P.l * kN, ring, p);
+}
+
+__device__ __forceinline__ void kp_secret_role(const KemParams &P, const uint8_t *seeds, uint8_t *sk, uint16_t *S,
+                                               int64_t t, uint32_t *ring, int p)
+{
+    uint32_t m[4];
+    load_words(m, seeds + t * 96 + 32, p);
+    secret_stream(P, m, S + t * P.l * kN, ring, p);
+    load_words(m, seeds + t * 96 + 64, p);
+    store_words(sk + t * P.sk + P.sk - 32, m, p);
+}
+
+// Encapsulation.  Matrix role: GenMatrix from the pk tail.  Secret role:
+// m = H(rnd), kr = G(m | H(pk)), the secret from kr[32:64], and the InnerProd
+// operands: b of the pk to V and m's bits to bits.  hpk (if not null) holds
+// H(pk) per instance.
+__device__ __forceinline__ void enc_matrix_role(const KemParams &P, const uint8_t *pk, uint16_t *A, int64_t t,
+                                                uint32_t *ring, int p)
+{
+    uint32_t m[4];
+    load_words(m, pk + t * P.pk + P.pvc, p);
+    gen_matrix(P, m, A + t * P.l * P.l * kN, ring, p);
+}
+
+__device__ __forceinline__ void enc_secret_role(const KemParams &P, const uint8_t *rnd, const uint8_t *pk,
+                                                const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *S,
+                                                uint16_t *V, uint16_t *bits, int64_t t, uint32_t *ring, int p)
+{
+    uint32_t r[4], m[4], h[4], coins[4];
+    load_words(r, rnd + t * 32, p);
+    sha3_256_words(m, r, p);
+    if (hpk) load_words(h, hpk + t * 32, p);
+    else sha3_256(h, pk + t * P.pk, P.pk, p);
+    hash_g(m, h, buf + t * 64, kr + t * 64, coins, p);
+    secret_stream(P, coins, S + t * P.l * kN, ring, p);
+    message_bits(m, bits + t * kN, p);
+    unpack_b(P, pk + t * P.pk, V + t * P.l * kN, p);
+}
+
 __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
                                                       uint16_t *A, uint16_t *S, int64_t B)
 {
     TMVP_PAIR(0);
     TMVP_RING;
     if (t >= 2 * B) return;
-    uint32_t m[4];
-    if (t < B) {
-        uint32_t st[25];
-        load_words(m, seeds + t * 96, p);
-        absorb_words(st, m, kShake128Rate, kShakeDS, p);
-        f1600(st, p);
-        const uint32_t sa[4] = {st[0], st[1], st[2], st[3]};
-        store_words(pk + t * P.pk + P.pvc, sa, p);
-        store_words(sk + t * P.sk + P.indcpa_sk + P.pvc, sa, p);
-        gen_matrix(P, sa, A + t * P.l * P.l * kN, ring, p);
-    } else {
-        const int64_t i = t - B;
-        load_words(m, seeds + i * 96 + 32, p);
-        secret_stream(P, m, S + i * P.l * kN, ring, p);
-        load_words(m, seeds + i * 96 + 64, p);
-        store_words(sk + i * P.sk + P.sk - 32, m, p);
-    }
+    if (t < B) kp_matrix_role(P, seeds, pk, sk, A, t, ring, p);
+    else kp_secret_role(P, seeds, sk, S, t - B, ring, p);
 }
 
-// Encapsulation: matrix from the pk tail; m = H(rnd), kr = G(m | H(pk)),
-// secret from kr[32:64], and the InnerProd operands: b of the pk to V and m's
-// bits to bits.  hpk (if not null) holds H(pk) per instance.
 __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
                                                        const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *A,
                                                        uint16_t *S, uint16_t *V, uint16_t *bits,
@@ -316,22 +348,8 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
     TMVP_PAIR(t0);
     TMVP_RING;
     if (t >= 2 * B) return;
-    uint32_t m[4];
-    if (t < B) {
-        load_words(m, pk + t * P.pk + P.pvc, p);
-        gen_matrix(P, m, A + t * P.l * P.l * kN, ring, p);
-    } else {
-        const int64_t i = t - B;
-        uint32_t r[4], h[4], coins[4];
-        load_words(r, rnd + i * 32, p);
-        sha3_256_words(m, r, p);
-        if (hpk) load_words(h, hpk + i * 32, p);
-        else sha3_256(h, pk + i * P.pk, P.pk, p);
-        hash_g(m, h, buf + i * 64, kr + i * 64, coins, p);
-        secret_stream(P, coins, S + i * P.l * kN, ring, p);
-        message_bits(m, bits + i * kN, p);
-        unpack_b(P, pk + i * P.pk, V + i * P.l * kN, p);
-    }
+    if (t < B) enc_matrix_role(P, pk, A, t, ring, p);
+    else enc_secret_role(P, rnd, pk, hpk, buf, kr, S, V, bits, t - B, ring, p);
 }
 
 // Decapsulation, before the decryption and beside it (on the auxiliary
@@ -572,8 +590,8 @@ __global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint8_t 
 
 static inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
-static hipError_t product(const KemParams &P, int backend, int mode, int epi, uint16_t *out, const uint16_t *a,
-                          const uint16_t *s, const uint16_t *aux, int64_t B, int *status, hipStream_t st)
+static Job make_job(const KemParams &P, int mode, int epi, uint16_t *out, const uint16_t *a, const uint16_t *s,
+                    const uint16_t *aux, int64_t B, int *status)
 {
     Job J;
     memset(&J, 0, sizeof(J));
@@ -589,6 +607,13 @@ static hipError_t product(const KemParams &P, int backend, int mode, int epi, ui
     J.epi = epi;
     J.w = (mode == kInner) ? P.ep : P.eq;
     J.h = (epi == kEpiDec) ? P.h2 : P.h1;
+    return J;
+}
+
+static hipError_t product(const KemParams &P, int backend, int mode, int epi, uint16_t *out, const uint16_t *a,
+                          const uint16_t *s, const uint16_t *aux, int64_t B, int *status, hipStream_t st)
+{
+    const Job J = make_job(P, mode, epi, out, a, s, aux, B, status);
     return backend == 0 ? launch_mfma(J, mode, P.l, st) : launch_valu(J, mode, P.l, st);
 }
 

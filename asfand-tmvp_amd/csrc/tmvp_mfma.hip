@@ -1,386 +1,11 @@
-// MFMA form: exact negacyclic products on gfx950 int8 matrix cores.
-//
-// Block decomposition (n = 256 = 8 blocks of 32).  For c = T(s) a with
-// T(s)[m][k] = ts(m - k) (ts(d) = s[d] for d >= 0, -s[d + 256] for d < 0),
-// write m = 32R + i and k = 32C + jj.  The 32x32 block (R, C) of T(s) is
-// eps * B_D with D = (R - C) mod 8, B_D[i][jj] = ts(32D + i - jj) and
-// eps = -1 exactly when R < D (then C = R - D + 8 and the entry wraps once).
-// So, with the extended block list E[e] = a_{e-8} for e >= 8 and E[e] = -a_e
-// for e < 8,
-//
-//     c[32R + i] = sum_D sum_jj B_D[i][jj] * E[R - D + 8][jj],
-//
-// one 32 x 256 x 8 matrix product per (row, limb) with no redundant MACs and
-// no 256 x 256 matrix anywhere.  The reference instead materialises T in HBM
-// (S/src/poly.cu:1116-1129) and multiplies three padded 128 x 128 fp16 tiles
-// (S/src/poly.cu:1155-1205), which is inexact for q = 2^13.
-//
-// Exactness.  The secret is used centred mod 2^w as an int8 (CBD values are
-// at most 5 in magnitude; for 8 <= w that int8 is simply the low byte).  The
-// public coefficient (mod 2^w, 8 <= w <= 14) is split into limbs a = lo + 128 hi,
-// lo and hi in [0, 127]; the negated copies stay inside int8.  Each limb
-// product accumulates exactly in int32 (|sum| <= 4*256*127*127 < 2^31) and
-// out = lo + 128 hi is exact mod 2^32.
-//
-// Mapping onto v_mfma_i32_16x16x64_i8 (M=16, N=16, K=64):
-//   M = 16 rows i of one half (h) of a 32-row block row,
-//   K = 64 = two consecutive D tiles x 32 jj (four K-chunks p cover D = 0..7),
-//   N = 16 = 8 block columns R x 2 limbs.
-// The A operand (Toeplitz tiles of s) is loaded once per secret polynomial
-// into 32 VGPRs and reused by every row and both limbs; the B operand is one
-// aligned 16-byte LDS read per MFMA pair (it feeds both halves h).
-//
-// An A-operand lane needs 16 consecutive band bytes at an offset whose
-// misalignment (i + 1) mod 4 is fixed per lane.  The band is therefore stored
-// four times, shifted by 0..3 bytes, and every lane reads a dword-aligned
-// window of its own copy (unaligned LDS loads are exact on gfx950 but many
-// times slower).  K indices run in reversed jj order inside each 16-byte group
-// (the E blocks are stored byte-reversed) so that these windows ascend in
-// memory.  The LDS serves a two-dword read one dword and 32 lanes at a time
-// from 32 banks: in each half-wave, each copy's 8 lanes read 8 consecutive
-// dwords, and the copies sit at 288 c (+ 4 for c > 0) bytes so that the four
-// 8-dword windows fall in distinct banks (at a plain 288-byte stride every
-// pass has a two-way conflict).
-//
-// Launch: persistent, 256 threads = 4 waves per workgroup.  A wave owns one
-// instance at a time in a private LDS slice, stages it one secret term at a
-// time, walks the batch with a grid stride, and loads each term of the next
-// instance into registers as soon as the current one has been staged.
+// MFMA form: the MatrixVectorMul / InnerProd / polymul launches (single job
+// and the mixed persistent launch).  The block decomposition, the operand
+// layouts and the per-wave work (mfma_body) are in tmvp_mfma_body.h.
 #include <stdlib.h>
 
-#include "tmvp_internal.h"
+#include "tmvp_mfma_body.h"
 
 namespace tmvp {
-
-typedef int v4i __attribute__((ext_vector_type(4)));
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-// Per-halfword logical right shift of a packed uint16 pair.
-__device__ __forceinline__ uint32_t shr16x2(uint32_t a, int n)
-{
-    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) >> (unsigned short)n);
-}
-// Per-halfword add of two packed uint16 pairs (wraps mod 2^16).
-__device__ __forceinline__ uint32_t add16x2(uint32_t a, uint32_t b)
-{
-    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) + __builtin_bit_cast(u16x2, b));
-}
-
-constexpr int kEBlock = 32;           // bytes per E block (32 coefficients, one limb)
-constexpr int kEPoly = 16 * kEBlock;  // blocks e = 0..15 (e = 0 unused)
-constexpr int kBand = 288;            // bytes of one band copy: X[x] = ts(x - 32), x < 288
-// Byte offset of band copy c (see the bank note above); the four copies take kBandBytes.
-__host__ __device__ constexpr int band_copy(int c) { return 288 * c + (c ? 4 : 0); }
-constexpr int kBandBytes = 1168; // band_copy(3) + kBand, rounded up to 16
-static_assert(band_copy(3) + kBand <= kBandBytes && kBandBytes % 16 == 0, "band copies");
-
-// Bytewise negation of four bytes that are each <= 0x80.
-__device__ __forceinline__ uint32_t neg_small(uint32_t v) { return (0x80808080u - v) ^ 0x80808080u; }
-// Bytewise two's-complement negation of four arbitrary bytes.
-__device__ __forceinline__ uint32_t neg_bytes(uint32_t v)
-{
-    const uint32_t y = ~v;
-    return ((y & 0x7f7f7f7fu) + 0x01010101u) ^ (y & 0x80808080u);
-}
-__device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) { return __builtin_amdgcn_perm(hi, lo, sel); }
-
-// Per-halfword helpers on packed uint16 pairs (arithmetic mod 2^16).
-__device__ __forceinline__ uint32_t shl16x2(uint32_t a, uint32_t n)
-{
-    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) << __builtin_bit_cast(u16x2, n));
-}
-__device__ __forceinline__ uint32_t sub16x2(uint32_t a, uint32_t b)
-{
-    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
-}
-
-// Epilogue on two outputs at once: y holds (exact sum + h) mod 2^16 for two
-// rows (h, the rounding constant, enters through the accumulator), `old` the
-// current outputs (accumulate) and `aux` the two aux words.
-template <int EPI>
-__device__ __forceinline__ uint32_t epi2(const Job &J, uint32_t y, uint32_t old, uint32_t aux)
-{
-    const uint32_t two = 0x00010001u;
-    if (EPI == kEpiQ2P) return shr16x2(y, J.eq - J.ep) & (((1u << J.ep) - 1u) * two);
-    if (EPI == kEpiEnc) {
-        const uint32_t m = shl16x2(aux & two, (uint32_t)(J.ep - 1) * two);
-        return shr16x2(sub16x2(y, m), J.ep - J.et) & (((1u << J.et) - 1u) * two);
-    }
-    if (EPI == kEpiDec) {
-        const uint32_t c = shl16x2(aux & (((1u << J.et) - 1u) * two), (uint32_t)(J.ep - J.et) * two);
-        return shr16x2(sub16x2(y, c), J.ep - 1) & two;
-    }
-    return (J.accumulate ? add16x2(y, old) : y) & (((1u << J.w) - 1u) * two);
-}
-
-// Per-wave LDS slice: the E blocks of every row for ONE secret term j, and the
-// 4 shifted copies of that term's band.  Terms are staged one after another,
-// so the staging of term j+1 can overlap the matrix-core work of term j.
-template <int L, int MODE> struct Slice {
-    static constexpr int NR = rows_of(MODE, L);
-    static constexpr int NT = terms_of(MODE, L);
-    static constexpr int NPOLY = NR * NT;       // public polynomials per instance
-    static constexpr int CJ = NR * 32;          // 8-coefficient chunks of one term
-    static constexpr int ITJ = (CJ + 63) / 64;  // staging loads per lane per term
-    static constexpr int E_BYTES = NR * 2 * kEPoly;
-    static constexpr int BYTES = E_BYTES + kBandBytes;
-};
-
-// DPP wave_shl:1 - lane i receives lane i + 1 (lane 63 receives 0).
-__device__ __forceinline__ uint32_t from_next_lane(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
-}
-
-// Register budget: 3 waves per SIMD (168 VGPRs) up to three rows and terms,
-// 2 waves per SIMD for the four-row FireSaber / Sable-3 shapes.
-template <int L, int MODE> constexpr int kWavesPerEU = (rows_of(MODE, L) * terms_of(MODE, L) > 9) ? 2 : 3;
-
-// LDS of one workgroup (four waves, two slices each).
-template <int L, int MODE> constexpr int kLdsBytes = 4 * 2 * Slice<L, MODE>::BYTES;
-
-// The work of one wave: instances first, first + stride, ... of job J, with
-// the wave's two LDS slices at lds + wave * 2 * Slice::BYTES.
-template <int L, int MODE, int EPI>
-__device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t stride, uint8_t *lds)
-{
-    using S = Slice<L, MODE>;
-    constexpr int NR = S::NR, NT = S::NT, NPOLY = S::NPOLY, CJ = S::CJ, ITJ = S::ITJ;
-    // The wave index is uniform; readfirstlane lets the compiler keep the
-    // instance loop (addresses, bounds, branches) in scalar registers.
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    int64_t inst = first;
-    if (inst >= J.batch) return; // wave-uniform: each wave owns private LDS slices, no barriers
-    uint8_t *const slice0 = lds + wave * 2 * S::BYTES;
-    int buf = 0; // slice holding the term about to be read
-    const uint32_t wmask = (1u << J.w) - 1u;
-    const uint32_t lo4 = 0x7f7f7f7fu;                 // lo limb mask of four gathered bytes
-    const uint32_t hi4 = (wmask >> 7) * 0x01010101u;  // hi limb mask of four gathered bytes (w >= 8)
-    const uint32_t wm2 = wmask * 0x00010001u;
-
-    // lane roles
-    const int blk = (lane & 31) >> 2;     // staging: 32-coefficient block of the lane's chunk
-    const int bpos = 24 - 8 * (lane & 3); // staging: byte position inside the reversed block
-    const int il = lane & 15, g = lane >> 4;
-    const int R = lane & 7, limb = (lane >> 3) & 1;
-    const int sh = (il + 1) & 3;
-    // A operand: dword index of the window start x0 - sh in band copy `sh`, without
-    // the compile-time part 16p + 4h (x0 = 32D + 16h - 16(g&1) + il + 17, D = 2p + (g>>1);
-    // il + 17 - sh is a multiple of 4).
-    const int abase = band_copy(sh) / 4 + 8 * (g >> 1) - 4 * (g & 1) + ((il + 17 - sh) >> 2);
-    // B operand: byte offset of E[R - (g>>1) + 8] (minus 64 p per K-chunk) in the lane's limb plane
-    const int bbase = limb * kEPoly + 16 * (1 - (g & 1)) + (R - (g >> 1) + 8) * kEBlock;
-
-    // Registers holding one instance's inputs, term by term.
-    u32x4 va[NT][ITJ];
-    u32x2 vs[NT];
-    // Loads of term j of instance i.  Every load is issued on every path (an
-    // instance past the end reloads the current one, lanes without a chunk
-    // reload another chunk of the same term and ignore it), so the compiler
-    // sees one fixed sequence of memory operations per iteration and can wait
-    // for exactly the loads it needs; conditional loads make it wait for all.
-    auto fetch_term = [&](int64_t i, int j) {
-        const int64_t ie = i < J.batch ? i : inst;
-        const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + ie * (NPOLY * kN));
-#pragma unroll
-        for (int it = 0; it < ITJ; it++) {
-            const int u0 = lane + 64 * it;                             // chunk of this term: row u >> 5, piece u & 31
-            const int u = (CJ % 64 == 0 || u0 < CJ) ? u0 : u0 - 64;    // always a chunk of this term
-            const int r = u >> 5;
-            const int pq = (MODE == kMatVec) ? r * L + j : (MODE == kMatVecT) ? j * L + r : j;
-            va[j][it] = src[pq * 32 + (u & 31)];
-        }
-        vs[j] = reinterpret_cast<const u32x2 *>(J.s + ie * (NT * kN))[j * 64 + lane];
-    };
-#pragma unroll
-    for (int j = 0; j < NT; j++) fetch_term(inst, j);
-
-    uint32_t bad = 0;
-    // Stage term j from va[j] / vs[j] into slice `into`.
-    auto stage = [&](int j, int into) {
-        uint8_t *const E = slice0 + into * S::BYTES;
-        uint8_t *const X = E + S::E_BYTES;
-        // ---- public polynomials of term j -> byte-reversed limb blocks E[r][limb][e]
-#pragma unroll
-        for (int it = 0; it < ITJ; it++) {
-            const int u = lane + 64 * it;
-            if (CJ % 64 != 0 && u >= CJ) continue;
-            const int r = u >> 5;
-            const u32x4 v = va[j][it];
-            // hi limb = coefficient >> 7, per 16-bit half
-            const uint32_t s0 = shr16x2(v.x, 7), s1 = shr16x2(v.y, 7), s2 = shr16x2(v.z, 7), s3 = shr16x2(v.w, 7);
-            // gather the low byte of every coefficient, coefficient u of the chunk to byte 7 - u
-            const uint2 plo = make_uint2(perm(v.w, v.z, 0x00020406u) & lo4, perm(v.y, v.x, 0x00020406u) & lo4);
-            const uint2 phi = make_uint2(perm(s3, s2, 0x00020406u) & hi4, perm(s1, s0, 0x00020406u) & hi4);
-            uint8_t *slot = E + r * 2 * kEPoly + bpos;
-            *(uint2 *)(slot + (blk + 8) * kEBlock) = plo;
-            *(uint2 *)(slot + kEPoly + (blk + 8) * kEBlock) = phi;
-            // negated copy; for blk = 0 it lands in block 0, which no MFMA reads
-            *(uint2 *)(slot + blk * kEBlock) = make_uint2(neg_small(plo.x), neg_small(plo.y));
-            *(uint2 *)(slot + kEPoly + blk * kEBlock) = make_uint2(neg_small(phi.x), neg_small(phi.y));
-        }
-        // ---- secret s_j -> int8 band X[x] = ts(x - 32), stored 4x shifted: copy c
-        //      holds X[x + c] at byte x.  For w >= 8 the centred int8 is the low byte.
-        const u32x2 sv = vs[j];
-        const uint32_t pos = perm(sv.y, sv.x, 0x06040200u);
-        // range check: centred value in [-127, 127]  <=>  ((s + 127) mod 2^w) + 1 <= 255
-        bad |= (add16x2(add16x2(sv.x, 0x007f007fu) & wm2, 0x00010001u) |
-                add16x2(add16x2(sv.y, 0x007f007fu) & wm2, 0x00010001u));
-        const uint32_t neg = neg_bytes(pos);
-        // Band dword k: k = lane + 8 is `pos` of this lane, k = lane - 56 (lanes
-        // 56..63) is `neg`; dword 72 is zero padding.
-        const uint32_t p1 = from_next_lane(pos);
-        uint32_t n1 = from_next_lane(neg);
-        if (lane == 63) n1 = __builtin_amdgcn_readlane(pos, 0); // dword 8 follows dword 7
-        uint8_t *band = X + 32 + 4 * lane;
-        *(uint32_t *)band = pos;
-#pragma unroll
-        for (int c = 1; c < 4; c++) *(uint32_t *)(band + band_copy(c)) = __builtin_amdgcn_alignbyte(p1, pos, c);
-        if (lane >= 56) {
-            uint8_t *nb = X + 4 * lane - 224;
-            *(uint32_t *)nb = neg;
-#pragma unroll
-            for (int c = 1; c < 4; c++) *(uint32_t *)(nb + band_copy(c)) = __builtin_amdgcn_alignbyte(n1, neg, c);
-        }
-    };
-    // The slices are private to this wave and the LDS completes one wave's
-    // operations in order: a wave-level fence at the end of every term keeps
-    // the compiler from moving a term's staging writes past its reads, or the
-    // reads of a slice past the staging that later overwrites it.
-    auto wave_fence = [] {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
-
-    // Every use of a term's input registers must come before the prefetch that
-    // reloads them: otherwise the compiler gives the load another register,
-    // copies it back at the loop edge and waits for it there (vmcnt), which
-    // exposes the full HBM latency once per instance.  The staging's LDS
-    // writes are ordered by the memory clobber; its one pure-register result,
-    // the range-check word, is routed through the barrier.
-    auto pin_after_stage = [&] { asm volatile("" : "+v"(bad) : : "memory"); };
-
-    // Software pipeline: the staging of the following term (the next term of
-    // this instance, or term 0 of the next instance) goes to the other slice
-    // and is issued between the MFMAs of the current term.
-    stage(0, 0);
-    pin_after_stage();
-    fetch_term(inst + stride, 0);
-    wave_fence();
-    // Accumulators start at the rounding constant h in the lo-limb lanes, so
-    // the epilogue never adds it.
-    const int hinit = (EPI != kEpiMod && limb == 0) ? J.h : 0;
-    const v4i init = {hinit, hinit, hinit, hinit};
-    // Epilogue roles: lo-limb lanes finish rows v = 0, 1 of their group, hi-limb
-    // lanes rows v = 2, 3; each sends the partner the other pair.
-    const uint32_t lshift = limb ? 0x00070007u : 0u;
-    for (;;) {
-        const int64_t next = inst + stride;
-        v4i acc[NR][2];
-#pragma unroll
-        for (int j = 0; j < NT; j++) {
-            const uint8_t *const E = slice0 + buf * S::BYTES;
-            // A operand: Toeplitz tiles of s_j for the four K-chunks and both halves,
-            // 16 bytes from a dword-aligned window each.  The window's byte offset is
-            // made opaque so that the 16 reads share one address register and carry
-            // their constant parts as instruction offsets.
-            int aoff = buf * S::BYTES + S::E_BYTES + 4 * abase;
-            asm volatile("" : "+v"(aoff));
-            const uint32_t *aw = reinterpret_cast<const uint32_t *>(slice0 + aoff);
-            // A fragments of one K-chunk (both halves); chunk p + 1 is read while
-            // chunk p is multiplied.
-            auto read_a = [&](int p, v4i (&f)[2]) {
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const uint32_t *w = aw + 16 * p + 4 * h;
-                    f[h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-                }
-            };
-            v4i acur[2];
-            read_a(0, acur);
-            // B operand of K-chunk 0; chunk p + 1 is read while chunk p is multiplied.
-            v4i bcur[NR];
-#pragma unroll
-            for (int r = 0; r < NR; r++) bcur[r] = *(const v4i *)(E + r * 2 * kEPoly + bbase);
-
-            if (j + 1 < NT) {
-                stage(j + 1, buf ^ 1);
-                pin_after_stage();
-                fetch_term(next, j + 1);
-            } else if (next < J.batch) {
-                stage(0, buf ^ 1); // va[0] / vs[0] already hold the next instance's term 0
-                pin_after_stage();
-                fetch_term(next + stride, 0);
-            }
-
-            // K-chunk outermost: 2*NR independent accumulators between dependent MFMAs.
-#pragma unroll
-            for (int p = 0; p < 4; p++) {
-                v4i bnxt[NR], anxt[2];
-                if (p < 3) {
-                    read_a(p + 1, anxt);
-#pragma unroll
-                    for (int r = 0; r < NR; r++) bnxt[r] = *(const v4i *)(E + r * 2 * kEPoly + bbase - 64 * (p + 1));
-                }
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    const bool first = (j == 0 && p == 0);
-                    acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur[0], bcur[r], first ? init : acc[r][0], 0, 0, 0);
-                    acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur[1], bcur[r], first ? init : acc[r][1], 0, 0, 0);
-                }
-                if (p < 3) {
-                    acur[0] = anxt[0];
-                    acur[1] = anxt[1];
-#pragma unroll
-                    for (int r = 0; r < NR; r++) bcur[r] = bnxt[r];
-                }
-            }
-            // Schedule: chunk 0's fragment reads first, then per chunk the next
-            // chunk's A and B reads and one MFMA per three staging VALU instructions.
-            __builtin_amdgcn_sched_group_barrier(0x100, 4 + NR, 0);
-#pragma unroll
-            for (int p = 0; p < 4; p++) {
-                if (p < 3) __builtin_amdgcn_sched_group_barrier(0x100, 4 + NR, 0);
-#pragma unroll
-                for (int k = 0; k < 2 * NR; k++) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
-                }
-            }
-            buf ^= 1;
-            wave_fence();
-        }
-
-        // ---- epilogue, in packed 16-bit arithmetic (every output is mod 2^w,
-        //      w <= 14): lane (R, limb, g) holds rows i = 16h + 4g + v of block
-        //      column R for its limb.  Both lanes of a pair scale their own limb
-        //      (hi by 128), keep one row pair and send the other across
-        //      row_ror:8, so each finishes two rows: lo lanes v = 0,1, hi v = 2,3.
-        uint16_t *obase = J.out + inst * (NR * kN) + 32 * R + 4 * g + 2 * limb;
-        const uint16_t *xbase = (EPI == kEpiEnc || EPI == kEpiDec) ? J.aux + inst * kN + 32 * R + 4 * g + 2 * limb : nullptr;
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const uint32_t p01 = shl16x2(perm((uint32_t)acc[r][h][1], (uint32_t)acc[r][h][0], 0x05040100u), lshift);
-                const uint32_t p23 = shl16x2(perm((uint32_t)acc[r][h][3], (uint32_t)acc[r][h][2], 0x05040100u), lshift);
-                const uint32_t keep = limb ? p23 : p01;
-                const uint32_t send = limb ? p01 : p23;
-                const uint32_t y = add16x2(keep, (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0x128, 0xf, 0xf, false));
-                uint32_t old = 0u, auxv = 0u;
-                if (EPI == kEpiMod && J.accumulate) old = *(const uint32_t *)(obase + r * kN + 16 * h);
-                if (EPI == kEpiEnc || EPI == kEpiDec) auxv = *(const uint32_t *)(xbase + 16 * h);
-                *(uint32_t *)(obase + r * kN + 16 * h) = epi2<EPI>(J, y, old, auxv);
-            }
-        }
-        inst = next;
-        if (inst >= J.batch) break;
-    }
-    if (bad & 0xff00ff00u) *J.status = 1;
-}
 
 template <int L, int MODE, int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEU<L, MODE>))) void mfma_kernel(Job J)
