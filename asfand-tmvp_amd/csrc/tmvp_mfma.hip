@@ -104,9 +104,9 @@ template <int K> __device__ __forceinline__ void run_kind(const Job &J, int64_t 
 } // namespace
 
 // Workgroup b belongs to the segment whose range holds b; its four waves take
-// instances first, first + stride, ... of that segment's job.  Two waves per
-// SIMD: the register budget of the largest shape.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mfma_multi_kernel(MultiJob M)
+// instances first, first + stride, ... of that segment's job.  Three waves per
+// SIMD: the register budget of the largest shape (148 VGPRs).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void mfma_multi_kernel(MultiJob M)
 {
     __shared__ __attribute__((aligned(16))) uint8_t lds[kMultiLds];
     int k = 0;

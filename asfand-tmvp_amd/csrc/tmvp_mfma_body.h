@@ -137,9 +137,10 @@ __device__ __forceinline__ uint32_t from_next_lane(uint32_t v)
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
 }
 
-// Register budget: 3 waves per SIMD (168 VGPRs) up to three rows and terms,
-// 2 waves per SIMD for the four-row FireSaber / Sable-3 shapes.
-template <int L, int MODE> constexpr int kWavesPerEU = (rows_of(MODE, L) * terms_of(MODE, L) > 9) ? 2 : 3;
+// Register budget: at least 3 waves per SIMD (168 VGPRs) for every shape; the
+// A fragments are read one K-chunk ahead, which keeps even the four-row
+// FireSaber / Sable-3 shapes near 150 VGPRs.
+template <int L, int MODE> constexpr int kWavesPerEU = 3;
 
 // LDS of one workgroup (four waves, two slices each).
 template <int L, int MODE> constexpr int kLdsBytes = 4 * 2 * Slice<L, MODE>::BYTES;
