@@ -189,8 +189,9 @@ int tmvp_graph_end(tmvp_ctx *ctx, tmvp_stream_t stream, tmvp_graph **graph);
 int tmvp_graph_launch(tmvp_graph *graph, tmvp_stream_t stream);
 int tmvp_graph_destroy(tmvp_graph *graph);
 
-/* ---- batched KEM (fused GPU pipeline: SHAKE-128 GenMatrix, CBD GenSecret,
- * MatrixVectorMul, InnerProd, rounding, codecs, Fujisaki-Okamoto hashes) ----
+/* ---- batched KEM (a GPU chain of fused kernels: SHAKE-128 GenMatrix and CBD
+ * GenSecret in the sponges, MatrixVectorMul / InnerProd with their rounding,
+ * codecs, Fujisaki-Okamoto hashes) ----
  * Byte layouts are the reference's: pk = pack_EP(b) | seed_A,
  * sk = s | pk | H(pk) | z (Saber s as 13-bit words, Sable as 2-bit codes),
  * ct = pack_EP(b') | pack_ET(v').  Buffers are [batch][size] device arrays.
