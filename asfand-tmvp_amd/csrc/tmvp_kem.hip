@@ -374,15 +374,16 @@ __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t 
 }
 
 // Decapsulation, after the decryption: m' from its bits, kr = G(m' | H(pk))
-// with H(pk) from the sk, secret from kr[32:64], and b of the pk inside the
-// sk to V for the re-encryption.
+// with H(pk) from the sk, secret from kr[32:64], b of the pk inside the sk to
+// V for the re-encryption, and the instance's mismatch flag cleared.
 __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
                                                        uint8_t *buf, uint8_t *kr, uint16_t *S, uint16_t *V,
-                                                       int64_t B)
+                                                       uint32_t *diff, int64_t B)
 {
     TMVP_PAIR(0);
     TMVP_RING;
     if (t >= B) return;
+    if (p == 0) diff[t] = 0; // set by the re-encryption's compare-packs below
     uint32_t m[4];
     // this lane's 32 bits of each message word: bits 64w + 32p + j
     const uint32_t *mb = reinterpret_cast<const uint32_t *>(mbits + t * kN + 32 * p); // bit k in uint16 k
@@ -414,11 +415,13 @@ __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16
 enum Xf : int { kXfNone = 0, kXfSk2 = 1 };
 
 // With dst2 set, the packed words also go to dst2 + t * dst2_stride (+ the
-// same chunk offset).
+// same chunk offset).  With diff set, nothing is written: the packed words are
+// compared with those already at dst, and a mismatch sets diff[t] to 1
+// (decapsulation's check of the re-encryption against the ciphertext).
 template <int W, int XF>
 __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t src_stride, uint8_t *dst,
                                                    int64_t dst_stride, int dst_off, int chunks, int64_t B,
-                                                   uint8_t *dst2, int64_t dst2_stride)
+                                                   uint8_t *dst2, int64_t dst2_stride, uint32_t *diff)
 {
     const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= B * chunks) return;
@@ -443,6 +446,13 @@ __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t 
         }
     }
     uint64_t *d = reinterpret_cast<uint64_t *>(dst + t * dst_stride + dst_off + (int64_t)c * W * 8);
+    if (diff) {
+        uint64_t x = 0;
+#pragma unroll
+        for (int k = 0; k < W; k++) x |= d[k] ^ words[k];
+        if (x) atomicOr(diff + t, 1u);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < W; k++) d[k] = words[k];
     if (dst2) {
@@ -546,32 +556,17 @@ __global__ __launch_bounds__(64) void enc_hash2_kernel(KemParams P, const uint8_
     store_words(ss + t * 32, s, p);
 }
 
-// Decapsulation, last step: compare ct with the re-encryption, kr[32:] = H(ct)
-// (computed beforehand), replace kr[0:32] by z on mismatch, ss = H(kr).
-// fail[t] records the outcome.
-__global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint8_t *ct, const uint8_t *ctp,
-                                                     const uint8_t *sk, const uint8_t *hct, const uint8_t *kr,
-                                                     uint8_t *ss, uint8_t *fail, int64_t B)
+// Decapsulation, last step: diff[t] is the outcome of comparing ct with the
+// re-encryption (set while packing it), kr[32:] = H(ct) (computed
+// beforehand); replace kr[0:32] by z on mismatch, ss = H(kr).  fail[t]
+// records the outcome.
+__global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint32_t *diff, const uint8_t *sk,
+                                                     const uint8_t *hct, const uint8_t *kr, uint8_t *ss, uint8_t *fail,
+                                                     int64_t B)
 {
     TMVP_PAIR(0);
     if (t >= B) return;
-    const uint32_t *a = reinterpret_cast<const uint32_t *>(ct + t * P.ct) + p;
-    const uint32_t *b = reinterpret_cast<const uint32_t *>(ctp + t * P.ct) + p;
-    uint32_t diff = 0;
-    int i = 0;
-    for (; i + 8 <= P.ct / 8; i += 8) { // eight words' loads in flight at a time
-        uint32_t x[8], y[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            x[u] = a[2 * (i + u)];
-            y[u] = b[2 * (i + u)];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) diff |= x[u] ^ y[u];
-    }
-    for (; i < P.ct / 8; i++) diff |= a[2 * i] ^ b[2 * i];
-    diff |= partner(diff);
-    const uint32_t mask = 0u - (uint32_t)(diff != 0); // all ones on mismatch, in both lanes
+    const uint32_t mask = 0u - (uint32_t)(diff[t] != 0); // all ones on mismatch, in both lanes
     uint32_t h[4], k[8], s[4], k0[4], z[4];
     load_words(h, hct + t * 32, p);
     load_words(k0, kr + t * 64, p);
@@ -628,20 +623,22 @@ static hipError_t product(const KemParams &P, int backend, int mode, int epi, ui
 // Codec launch for a run-time width: finds the instantiation with W == w.
 template <int W>
 static hipError_t pack_w(const uint16_t *src, int64_t src_stride, uint8_t *dst, int64_t dst_stride, int dst_off, int w,
-                         int chunks, int xf, int64_t B, hipStream_t st, uint8_t *dst2, int64_t dst2_stride)
+                         int chunks, int xf, int64_t B, hipStream_t st, uint8_t *dst2, int64_t dst2_stride,
+                         uint32_t *diff)
 {
     if constexpr (W > 13) {
         return hipErrorInvalidValue;
     } else {
         if (w != W)
-            return pack_w<W + 1>(src, src_stride, dst, dst_stride, dst_off, w, chunks, xf, B, st, dst2, dst2_stride);
+            return pack_w<W + 1>(src, src_stride, dst, dst_stride, dst_off, w, chunks, xf, B, st, dst2, dst2_stride,
+                                 diff);
         const dim3 grid(blocks_for(B * chunks, 256));
         if (xf == kXfSk2)
             hipLaunchKernelGGL((pack_kernel<W, kXfSk2>), grid, dim3(256), 0, st, src, src_stride, dst, dst_stride,
-                               dst_off, chunks, B, dst2, dst2_stride);
+                               dst_off, chunks, B, dst2, dst2_stride, diff);
         else
             hipLaunchKernelGGL((pack_kernel<W, kXfNone>), grid, dim3(256), 0, st, src, src_stride, dst, dst_stride,
-                               dst_off, chunks, B, dst2, dst2_stride);
+                               dst_off, chunks, B, dst2, dst2_stride, diff);
         return hipGetLastError();
     }
 }
@@ -668,9 +665,9 @@ static hipError_t unpack_w(const uint8_t *src, int64_t src_stride, int src_off, 
 // ncoef is a multiple of 64; byte offsets and strides are multiples of 8.
 static hipError_t pack(const uint16_t *src, int64_t src_stride, uint8_t *dst, int64_t dst_stride, int dst_off, int w,
                        int ncoef, int xf, int64_t B, hipStream_t st, uint8_t *dst2 = nullptr,
-                       int64_t dst2_stride = 0)
+                       int64_t dst2_stride = 0, uint32_t *diff = nullptr)
 {
-    return pack_w<1>(src, src_stride, dst, dst_stride, dst_off, w, ncoef / 64, xf, B, st, dst2, dst2_stride);
+    return pack_w<1>(src, src_stride, dst, dst_stride, dst_off, w, ncoef / 64, xf, B, st, dst2, dst2_stride, diff);
 }
 
 static hipError_t unpack(const uint8_t *src, int64_t src_stride, int src_off, uint16_t *dst, int64_t dst_stride,
@@ -687,20 +684,22 @@ static inline unsigned pair_blocks(int64_t n) { return blocks_for(2 * n, 64); }
 // bits (W.bits, one per coefficient: S/src/pack.cu:86-99), all written by the
 // front kernels, and writes pack_ET(round(b . s' - m)) to the ct tail; the
 // MatrixVectorMul half needs A and writes pack_EP(round(A s')) to the ct head.
-static hipError_t enc_inner_half(const KemParams &P, const KemWorkspace &W, int backend, uint8_t *ct, int64_t B,
-                                 int *status, hipStream_t st)
+// With diff set (decapsulation's re-encryption), the halves compare with the
+// ciphertext at ct instead of writing it.
+static hipError_t enc_inner_half(const KemParams &P, const KemWorkspace &W, int backend, uint8_t *ct,
+                                 uint32_t *diff, int64_t B, int *status, hipStream_t st)
 {
     TRY(product(P, backend, kInner, kEpiEnc, W.poly, W.v2, W.s, W.bits, B, status, st));
-    TRY(pack(W.poly, kN, ct, P.ct, P.pvc, P.et, kN, kXfNone, B, st));
+    TRY(pack(W.poly, kN, ct, P.ct, P.pvc, P.et, kN, kXfNone, B, st, nullptr, 0, diff));
     return hipSuccess;
 }
 
-static hipError_t enc_matvec_half(const KemParams &P, const KemWorkspace &W, int backend, uint8_t *ct, int64_t B,
-                                  int *status, hipStream_t st)
+static hipError_t enc_matvec_half(const KemParams &P, const KemWorkspace &W, int backend, uint8_t *ct,
+                                  uint32_t *diff, int64_t B, int *status, hipStream_t st)
 {
     const int L = P.l;
     TRY(product(P, backend, kMatVec, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
-    TRY(pack(W.v1, L * kN, ct, P.ct, 0, P.ep, L * kN, kXfNone, B, st));
+    TRY(pack(W.v1, L * kN, ct, P.ct, 0, P.ep, L * kN, kXfNone, B, st, nullptr, 0, diff));
     return hipSuccess;
 }
 
@@ -733,8 +732,8 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk, nullptr, W.buf, W.kr,
                        W.A, W.s, W.v2, W.bits, B, (int64_t)0);
     TRY(hipGetLastError());
-    TRY(enc_matvec_half(P, W, backend, ct, B, status, st));
-    TRY(enc_inner_half(P, W, backend, ct, B, status, st));
+    TRY(enc_matvec_half(P, W, backend, ct, nullptr, B, status, st));
+    TRY(enc_inner_half(P, W, backend, ct, nullptr, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
 }
@@ -765,12 +764,14 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     TRY(product(P, backend, kInner, kEpiDec, W.bits, W.v2, W.s, W.poly, B, status, st));
     // re-encryption of m' under the pk stored in the sk
     hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.bits, sk, W.buf, W.kr, W.s, W.v2,
-                       B);
+                       W.diff, B);
     TRY(hipGetLastError());
-    TRY(enc_inner_half(P, W, backend, W.ctp, B, status, st));
+    // ct is only read: the compare-packs never write through it
+    uint8_t *ctc = const_cast<uint8_t *>(ct);
+    TRY(enc_inner_half(P, W, backend, ctc, W.diff, B, status, st));
     TRY(hipStreamWaitEvent(st, W.join, 0));
-    TRY(enc_matvec_half(P, W, backend, W.ctp, B, status, st));
-    hipLaunchKernelGGL(dec_fin_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.ctp, sk, W.hct, W.kr, ss, fail,
+    TRY(enc_matvec_half(P, W, backend, ctc, W.diff, B, status, st));
+    hipLaunchKernelGGL(dec_fin_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.diff, sk, W.hct, W.kr, ss, fail,
                        B);
     return hipGetLastError();
 }
@@ -804,8 +805,8 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg, W.buf, W.kr,
                        W.A, W.s, W.v2, W.bits, B, B);
     TRY(hipGetLastError());
-    TRY(enc_inner_half(P, W, backend, ct, B, status, st));
-    TRY(enc_matvec_half(P, W, backend, ct, B, status, st));
+    TRY(enc_inner_half(P, W, backend, ct, nullptr, B, status, st));
+    TRY(enc_matvec_half(P, W, backend, ct, nullptr, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
 }
@@ -844,7 +845,7 @@ size_t kem_workspace_bytes(const KemParams &P, int64_t B)
     b += n * kN * 2;         // poly
     b += n * 64;             // buf
     b += n * 64;             // kr
-    b += n * (size_t)P.ct;   // ctp
+    b += n * 4;              // diff
     b += n * (size_t)P.pk;   // pkg
     b += n * (size_t)P.sk;   // skg
     b += n * 32;             // hpkg
@@ -868,7 +869,7 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
     W.poly = reinterpret_cast<uint16_t *>(take(n * kN * 2));
     W.buf = take(n * 64);
     W.kr = take(n * 64);
-    W.ctp = take(n * (size_t)P.ct);
+    W.diff = reinterpret_cast<uint32_t *>(take(n * 4));
     W.pkg = take(n * (size_t)P.pk);
     W.skg = take(n * (size_t)P.sk);
     W.hpkg = take(n * 32);
