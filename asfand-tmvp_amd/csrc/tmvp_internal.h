@@ -13,7 +13,9 @@ enum Mode : int {
     kPoly = 0,   // c = a * s                        (one product)
     kMatVec = 1, // out[i] = sum_j A[i][j] s[j]      (L rows x L terms)
     kMatVecT = 2,// out[i] = sum_j A[j][i] s[j]      (L rows x L terms)
-    kInner = 3   // v = sum_j b[j] s[j]              (1 row x L terms)
+    kInner = 3,  // v = sum_j b[j] s[j]              (1 row x L terms)
+    kEnc = 4     // encryption: out[i] = sum_j A[i][j] s[j] (q->p) and
+                 // out2 = sum_j b[j] s[j] (enc epilogue), L + 1 rows x L terms
 };
 
 // Fused epilogues applied to the exact sum x.
@@ -28,6 +30,8 @@ enum Epilogue : int {
 struct Job {
     uint16_t *out;
     const uint16_t *a;   // public operand (A, b or a)
+    const uint16_t *b;   // kEnc: b of the public key [batch][L][256]
+    uint16_t *out2;      // kEnc: the InnerProd row's output [batch][256]
     const uint16_t *s;   // secret operand
     const uint16_t *aux; // message bits / scale polynomial (epilogues 2, 3)
     int *status;         // set to 1 when an MFMA launch meets an out-of-range secret
@@ -39,7 +43,10 @@ struct Job {
     int eq, ep, et;
 };
 
-__host__ __device__ inline constexpr int rows_of(int mode, int l) { return (mode == kMatVec || mode == kMatVecT) ? l : 1; }
+__host__ __device__ inline constexpr int rows_of(int mode, int l)
+{
+    return (mode == kMatVec || mode == kMatVecT) ? l : mode == kEnc ? l + 1 : 1;
+}
 __host__ __device__ inline constexpr int terms_of(int mode, int l) { return mode == kPoly ? 1 : l; }
 
 // Epilogue on one coefficient.  x is the exact sum reduced mod 2^16 or mod

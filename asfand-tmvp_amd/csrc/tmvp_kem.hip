@@ -703,6 +703,25 @@ static hipError_t enc_matvec_half(const KemParams &P, const KemWorkspace &W, int
     return hipSuccess;
 }
 
+// The whole encryption.  In the MFMA form both products run as one: the L rows
+// of A and b as an extra row share every secret term's staging (kEnc).
+static hipError_t enc_both_halves(const KemParams &P, const KemWorkspace &W, int backend, uint8_t *ct, int64_t B,
+                                  int *status, hipStream_t st)
+{
+    if (backend != 0) {
+        TRY(enc_matvec_half(P, W, backend, ct, nullptr, B, status, st));
+        return enc_inner_half(P, W, backend, ct, nullptr, B, status, st);
+    }
+    const int L = P.l;
+    Job J = make_job(P, kMatVec, kEpiQ2P, W.v1, W.A, W.s, W.bits, B, status);
+    J.b = W.v2;
+    J.out2 = W.poly;
+    TRY(launch_mfma(J, kEnc, L, st));
+    TRY(pack(W.v1, L * kN, ct, P.ct, 0, P.ep, L * kN, kXfNone, B, st));
+    TRY(pack(W.poly, kN, ct, P.ct, P.pvc, P.et, kN, kXfNone, B, st));
+    return hipSuccess;
+}
+
 // Key generation and encapsulation run on one stream: the front kernel keeps
 // one sponge wave per SIMD, and the steps after it are short and mostly
 // bandwidth-bound, so running them beside each other gains less than the
@@ -732,8 +751,7 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk, nullptr, W.buf, W.kr,
                        W.A, W.s, W.v2, W.bits, B, (int64_t)0);
     TRY(hipGetLastError());
-    TRY(enc_matvec_half(P, W, backend, ct, nullptr, B, status, st));
-    TRY(enc_inner_half(P, W, backend, ct, nullptr, B, status, st));
+    TRY(enc_both_halves(P, W, backend, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
 }
@@ -805,8 +823,7 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
     hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg, W.buf, W.kr,
                        W.A, W.s, W.v2, W.bits, B, B);
     TRY(hipGetLastError());
-    TRY(enc_inner_half(P, W, backend, ct, nullptr, B, status, st));
-    TRY(enc_matvec_half(P, W, backend, ct, nullptr, B, status, st));
+    TRY(enc_both_halves(P, W, backend, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
 }

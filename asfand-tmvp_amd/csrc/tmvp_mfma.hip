@@ -74,6 +74,9 @@ hipError_t launch_mfma(const Job &J, int mode, int l, hipStream_t stream)
         if (J.epi == kEpiEnc) return by_l<kInner, kEpiEnc>(J, l, stream);
         if (J.epi == kEpiDec) return by_l<kInner, kEpiDec>(J, l, stream);
         break;
+    case kEnc:
+        if (J.epi == kEpiQ2P) return by_l<kEnc, kEpiQ2P>(J, l, stream);
+        break;
     }
     return hipErrorInvalidValue;
 }

@@ -137,10 +137,11 @@ __device__ __forceinline__ uint32_t from_next_lane(uint32_t v)
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
 }
 
-// Register budget: at least 3 waves per SIMD (168 VGPRs) for every shape; the
-// A fragments are read one K-chunk ahead, which keeps even the four-row
-// FireSaber / Sable-3 shapes near 150 VGPRs.
-template <int L, int MODE> constexpr int kWavesPerEU = 3;
+// Register budget: at least 3 waves per SIMD (168 VGPRs) up to 16 products per
+// instance; the A fragments are read one K-chunk ahead, which keeps even the
+// four-row FireSaber / Sable-3 shapes near 150 VGPRs.  The encryption shapes
+// (L + 1 rows) take 2, which they fill without spilling.
+template <int L, int MODE> constexpr int kWavesPerEU = (MODE == kEnc || rows_of(MODE, L) * terms_of(MODE, L) > 16) ? 2 : 3;
 
 // LDS of one workgroup (four waves, two slices each).
 template <int L, int MODE> constexpr int kLdsBytes = 4 * 2 * Slice<L, MODE>::BYTES;
@@ -185,16 +186,25 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     // reload another chunk of the same term and ignore it), so the compiler
     // sees one fixed sequence of memory operations per iteration and can wait
     // for exactly the loads it needs; conditional loads make it wait for all.
+    // Public polynomials per instance in J.a (kEnc: the L x L matrix; its row
+    // L is b, read from J.b).
+    constexpr int NPA = (MODE == kEnc) ? L * L : NPOLY;
     auto fetch_term = [&](int64_t i, int j) {
         const int64_t ie = i < J.batch ? i : inst;
-        const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + ie * (NPOLY * kN));
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + ie * (NPA * kN));
 #pragma unroll
         for (int it = 0; it < ITJ; it++) {
             const int u0 = lane + 64 * it;                             // chunk of this term: row u >> 5, piece u & 31
             const int u = (CJ % 64 == 0 || u0 < CJ) ? u0 : u0 - 64;    // always a chunk of this term
             const int r = u >> 5;
-            const int pq = (MODE == kMatVec) ? r * L + j : (MODE == kMatVecT) ? j * L + r : j;
-            va[j][it] = src[pq * 32 + (u & 31)];
+            if constexpr (MODE == kEnc) {
+                const u32x4 *row = r < L ? src + (r * L + j) * 32
+                                         : reinterpret_cast<const u32x4 *>(J.b + ie * (L * kN)) + j * 32;
+                va[j][it] = row[u & 31];
+            } else {
+                const int pq = (MODE == kMatVec) ? r * L + j : (MODE == kMatVecT) ? j * L + r : j;
+                va[j][it] = src[pq * 32 + (u & 31)];
+            }
         }
         vs[j] = reinterpret_cast<const u32x2 *>(J.s + ie * (NT * kN))[j * 64 + lane];
     };
@@ -363,8 +373,12 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
         //      column R for its limb.  Both lanes of a pair scale their own limb
         //      (hi by 128), keep one row pair and send the other across
         //      row_ror:8, so each finishes two rows: lo lanes v = 0,1, hi v = 2,3.
-        uint16_t *obase = J.out + inst * (NR * kN) + 32 * R + 4 * g + 2 * limb;
-        const uint16_t *xbase = (EPI == kEpiEnc || EPI == kEpiDec) ? J.aux + inst * kN + 32 * R + 4 * g + 2 * limb : nullptr;
+        //      kEnc: rows 0..L-1 go to J.out (q->p), row L to J.out2 (enc).
+        constexpr int NRO = (MODE == kEnc) ? L : NR; // rows in J.out per instance
+        const int lpos = 32 * R + 4 * g + 2 * limb;
+        uint16_t *obase = J.out + inst * (NRO * kN) + lpos;
+        const bool with_aux = EPI == kEpiEnc || EPI == kEpiDec || MODE == kEnc;
+        const uint16_t *xbase = with_aux ? J.aux + inst * kN + lpos : nullptr;
 #pragma unroll
         for (int r = 0; r < NR; r++) {
 #pragma unroll
@@ -374,6 +388,11 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
                 const uint32_t keep = limb ? p23 : p01;
                 const uint32_t send = limb ? p01 : p23;
                 const uint32_t y = add16x2(keep, (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0x128, 0xf, 0xf, false));
+                if (MODE == kEnc && r == L) {
+                    const uint32_t auxv = *(const uint32_t *)(xbase + 16 * h);
+                    *(uint32_t *)(J.out2 + inst * kN + lpos + 16 * h) = epi2<kEpiEnc>(J, y, 0u, auxv);
+                    continue;
+                }
                 uint32_t old = 0u, auxv = 0u;
                 if (EPI == kEpiMod && J.accumulate) old = *(const uint32_t *)(obase + r * kN + 16 * h);
                 if (EPI == kEpiEnc || EPI == kEpiDec) auxv = *(const uint32_t *)(xbase + 16 * h);
