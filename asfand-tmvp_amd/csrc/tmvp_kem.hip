@@ -35,6 +35,14 @@ using namespace keccak;
 // sponge kernel works on pair t = g >> 1, half p = g & 1, and both lanes of a
 // pair always take the same branch.
 
+// The compile-time shape of a parameter set's sampling: EQ (matrix bits), MU
+// (CBD width) and EP (bits of b).  Each sponge kernel is instantiated per
+// shape, so it holds only its own set's code paths (a kernel holding every
+// set's variants measured 19 us slower on Saber encapsulation).
+template <int EQ_, int MU_, int EP_> struct Shape {
+    static constexpr int EQ = EQ_, MU = MU_, EP = EP_;
+};
+
 #define TMVP_PAIR(t0)                                                                                                \
     const int64_t t = (((int64_t)blockIdx.x * 64 + threadIdx.x) >> 1) + (t0);                                       \
     const int p = (int)(threadIdx.x & 1)
@@ -91,10 +99,9 @@ template <int W> __device__ __forceinline__ void unpack_pair(const uint8_t *src,
 }
 
 // The vector b of a public key (its first pvc bytes, EP bits per coefficient).
-__device__ __forceinline__ void unpack_b(const KemParams &P, const uint8_t *pk, uint16_t *dst, int p)
+template <int EP> __device__ __forceinline__ void unpack_b(const KemParams &P, const uint8_t *pk, uint16_t *dst, int p)
 {
-    if (P.ep == 10) unpack_pair<10>(pk, dst, P.l * kN, p);
-    else unpack_pair<9>(pk, dst, P.l * kN, p);
+    unpack_pair<EP>(pk, dst, P.l * kN, p);
 }
 
 // The message bits as one 0/1 coefficient each (bit k of m is coefficient k):
@@ -167,11 +174,11 @@ __device__ __forceinline__ void matrix_to_a(const uint32_t (&seed)[4], uint16_t 
 }
 
 // A[t] from the 32-byte seed; `ring` is the instance's kRingDwords in LDS.
+template <int EQ>
 __device__ __forceinline__ void gen_matrix(const KemParams &P, const uint32_t (&seed)[4], uint16_t *arow,
                                            uint32_t *ring, int p)
 {
-    if (P.eq == 13) matrix_to_a<13>(seed, arow, P.araw / 8, ring, p);
-    else matrix_to_a<11>(seed, arow, P.araw / 8, ring, p);
+    matrix_to_a<EQ>(seed, arow, P.araw / 8, ring, p);
 }
 
 #define TMVP_RING                                                                                                    \
@@ -232,22 +239,22 @@ template <int MU> __device__ __forceinline__ void cbd32(const uint32_t (&d)[MU],
 }
 
 // The secret of one instance from its 32-byte seed, sampled into S.
+template <int MU>
 __device__ __forceinline__ void secret_stream(const KemParams &P, const uint32_t (&seed)[4], uint16_t *S,
                                               uint32_t *ring, int p)
 {
     const int nwords = P.sraw / 8;
     const uint32_t qmask = P.sable ? (1u << P.eq) - 1u : 0xffffu;
-    if (P.mu == 8 || P.mu == 2) {
+    if constexpr (MU == 8 || MU == 2) {
         uint32_t A[25];
         absorb_words(A, seed, kShake128Rate, kShakeDS, p);
-        if (P.mu == 8) squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd8_half(S, k, h, p, qmask); });
+        if constexpr (MU == 8) squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd8_half(S, k, h, p, qmask); });
         else squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd2_half(S, k, h, p, qmask); });
-    } else if (P.mu == 6) {
-        stream_groups<6, 5>(seed, nwords, ring, p,
-                            [&](int g, const uint32_t(&d)[6]) { cbd32<6>(d, S + 64 * g + 32 * p, qmask); });
     } else {
-        stream_groups<10, 3>(seed, nwords, ring, p,
-                             [&](int g, const uint32_t(&d)[10]) { cbd32<10>(d, S + 64 * g + 32 * p, qmask); });
+        static_assert(MU == 6 || MU == 10, "CBD width");
+        constexpr int RG = MU == 6 ? 5 : 3; // groups in the ring (at least 21 + MU - 1 words)
+        stream_groups<MU, RG>(seed, nwords, ring, p,
+                              [&](int g, const uint32_t(&d)[MU]) { cbd32<MU>(d, S + 64 * g + 32 * p, qmask); });
     }
 }
 
@@ -280,6 +287,7 @@ __device__ __forceinline__ void hash_g(const uint32_t (&m)[4], const uint32_t (&
 // Key generation: seeds[t] = seed_A | seed_s | z.  Matrix role: seed_A' =
 // SHAKE-128(seed_A) goes to the pk tail (and the sk's copy of it) and seeds
 // GenMatrix.  Secret role: seed_s seeds the secret; z goes to the sk tail.
+template <class SH>
 __device__ __forceinline__ void kp_matrix_role(const KemParams &P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
                                                uint16_t *A, int64_t t, uint32_t *ring, int p)
 {
@@ -290,15 +298,16 @@ __device__ __forceinline__ void kp_matrix_role(const KemParams &P, const uint8_t
     const uint32_t sa[4] = {st[0], st[1], st[2], st[3]};
     store_words(pk + t * P.pk + P.pvc, sa, p);
     store_words(sk + t * P.sk + P.indcpa_sk + P.pvc, sa, p);
-    gen_matrix(P, sa, A + t * P.l * P.l * kN, ring, p);
+    gen_matrix<SH::EQ>(P, sa, A + t * P.l * P.l * kN, ring, p);
 }
 
+template <class SH>
 __device__ __forceinline__ void kp_secret_role(const KemParams &P, const uint8_t *seeds, uint8_t *sk, uint16_t *S,
                                                int64_t t, uint32_t *ring, int p)
 {
     uint32_t m[4];
     load_words(m, seeds + t * 96 + 32, p);
-    secret_stream(P, m, S + t * P.l * kN, ring, p);
+    secret_stream<SH::MU>(P, m, S + t * P.l * kN, ring, p);
     load_words(m, seeds + t * 96 + 64, p);
     store_words(sk + t * P.sk + P.sk - 32, m, p);
 }
@@ -307,14 +316,16 @@ __device__ __forceinline__ void kp_secret_role(const KemParams &P, const uint8_t
 // m = H(rnd), kr = G(m | H(pk)), the secret from kr[32:64], and the InnerProd
 // operands: b of the pk to V and m's bits to bits.  hpk (if not null) holds
 // H(pk) per instance.
+template <class SH>
 __device__ __forceinline__ void enc_matrix_role(const KemParams &P, const uint8_t *pk, uint16_t *A, int64_t t,
                                                 uint32_t *ring, int p)
 {
     uint32_t m[4];
     load_words(m, pk + t * P.pk + P.pvc, p);
-    gen_matrix(P, m, A + t * P.l * P.l * kN, ring, p);
+    gen_matrix<SH::EQ>(P, m, A + t * P.l * P.l * kN, ring, p);
 }
 
+template <class SH>
 __device__ __forceinline__ void enc_secret_role(const KemParams &P, const uint8_t *rnd, const uint8_t *pk,
                                                 const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *S,
                                                 uint16_t *V, uint16_t *bits, int64_t t, uint32_t *ring, int p)
@@ -325,21 +336,23 @@ __device__ __forceinline__ void enc_secret_role(const KemParams &P, const uint8_
     if (hpk) load_words(h, hpk + t * 32, p);
     else sha3_256(h, pk + t * P.pk, P.pk, p);
     hash_g(m, h, buf + t * 64, kr + t * 64, coins, p);
-    secret_stream(P, coins, S + t * P.l * kN, ring, p);
+    secret_stream<SH::MU>(P, coins, S + t * P.l * kN, ring, p);
     message_bits(m, bits + t * kN, p);
-    unpack_b(P, pk + t * P.pk, V + t * P.l * kN, p);
+    unpack_b<SH::EP>(P, pk + t * P.pk, V + t * P.l * kN, p);
 }
 
+template <class SH>
 __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
                                                       uint16_t *A, uint16_t *S, int64_t B)
 {
     TMVP_PAIR(0);
     TMVP_RING;
     if (t >= 2 * B) return;
-    if (t < B) kp_matrix_role(P, seeds, pk, sk, A, t, ring, p);
-    else kp_secret_role(P, seeds, sk, S, t - B, ring, p);
+    if (t < B) kp_matrix_role<SH>(P, seeds, pk, sk, A, t, ring, p);
+    else kp_secret_role<SH>(P, seeds, sk, S, t - B, ring, p);
 }
 
+template <class SH>
 __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
                                                        const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *A,
                                                        uint16_t *S, uint16_t *V, uint16_t *bits,
@@ -348,13 +361,14 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
     TMVP_PAIR(t0);
     TMVP_RING;
     if (t >= 2 * B) return;
-    if (t < B) enc_matrix_role(P, pk, A, t, ring, p);
-    else enc_secret_role(P, rnd, pk, hpk, buf, kr, S, V, bits, t - B, ring, p);
+    if (t < B) enc_matrix_role<SH>(P, pk, A, t, ring, p);
+    else enc_secret_role<SH>(P, rnd, pk, hpk, buf, kr, S, V, bits, t - B, ring, p);
 }
 
 // Decapsulation, before the decryption and beside it (on the auxiliary
 // stream): the matrix stream from the pk inside the sk (pairs [t0, B); t0 = B
 // when A is resident) and H(ct) (pairs [B, 2B)).
+template <class SH>
 __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t *ct, const uint8_t *sk, uint16_t *A,
                                                      uint8_t *hct, int64_t B, int64_t t0)
 {
@@ -364,7 +378,7 @@ __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t 
     if (t < B) {
         uint32_t m[4];
         load_words(m, sk + t * P.sk + P.indcpa_sk + P.pvc, p);
-        gen_matrix(P, m, A + t * P.l * P.l * kN, ring, p);
+        gen_matrix<SH::EQ>(P, m, A + t * P.l * P.l * kN, ring, p);
     } else {
         const int64_t i = t - B;
         uint32_t h[4];
@@ -376,6 +390,7 @@ __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t 
 // Decapsulation, after the decryption: m' from its bits, kr = G(m' | H(pk))
 // with H(pk) from the sk, secret from kr[32:64], b of the pk inside the sk to
 // V for the re-encryption, and the instance's mismatch flag cleared.
+template <class SH>
 __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
                                                        uint8_t *buf, uint8_t *kr, uint16_t *S, uint16_t *V,
                                                        uint32_t *diff, int64_t B)
@@ -400,8 +415,8 @@ __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16
     uint32_t h[4], coins[4];
     load_words(h, sk + t * P.sk + P.sk - 64, p);
     hash_g(m, h, buf + t * 64, kr + t * 64, coins, p);
-    secret_stream(P, coins, S + t * P.l * kN, ring, p);
-    unpack_b(P, sk + t * P.sk + P.indcpa_sk, V + t * P.l * kN, p);
+    secret_stream<SH::MU>(P, coins, S + t * P.l * kN, ring, p);
+    unpack_b<SH::EP>(P, sk + t * P.sk + P.indcpa_sk, V + t * P.l * kN, p);
 }
 
 // ---------------------------------------------------------------- codecs
@@ -517,6 +532,7 @@ __global__ __launch_bounds__(64) void hash_pk_kernel(KemParams P, const uint8_t 
 }
 
 // GenMatrix from the 32-byte seed at seed + t * stride (for a key store).
+template <class SH>
 __global__ __launch_bounds__(64) void matrix_kernel(KemParams P, const uint8_t *seed, int64_t stride, uint16_t *A,
                                                     int64_t B)
 {
@@ -525,7 +541,7 @@ __global__ __launch_bounds__(64) void matrix_kernel(KemParams P, const uint8_t *
     if (t >= B) return;
     uint32_t m[4];
     load_words(m, seed + t * stride, p);
-    gen_matrix(P, m, A + t * P.l * P.l * kN, ring, p);
+    gen_matrix<SH::EQ>(P, m, A + t * P.l * P.l * kN, ring, p);
 }
 
 // Key generation, last step: H(pk) into the sk.
@@ -676,6 +692,20 @@ static hipError_t unpack(const uint8_t *src, int64_t src_stride, int src_off, ui
     return unpack_w<1>(src, src_stride, src_off, dst, dst_stride, w, ncoef / 64, xf, B, st);
 }
 
+// Calls f(Shape<EQ, MU, EP>{}) for the parameter set's sampling shape.
+template <class F> static hipError_t by_shape(const KemParams &P, F f)
+{
+    if (P.eq == 13 && P.ep == 10) {
+        if (P.mu == 8) return f(Shape<13, 8, 10>{});
+        if (P.mu == 10) return f(Shape<13, 10, 10>{});
+        if (P.mu == 6) return f(Shape<13, 6, 10>{});
+    } else if (P.eq == 11 && P.mu == 2) {
+        if (P.ep == 9) return f(Shape<11, 2, 9>{});
+        if (P.ep == 10) return f(Shape<11, 2, 10>{});
+    }
+    return hipErrorInvalidValue;
+}
+
 // Threads for n lane pairs.
 static inline unsigned pair_blocks(int64_t n) { return blocks_for(2 * n, 64); }
 
@@ -731,8 +761,11 @@ hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, c
 {
     if (B <= 0) return hipSuccess;
     const int L = P.l;
-    hipLaunchKernelGGL(kp_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk, W.A, W.s, B);
-    TRY(hipGetLastError());
+    TRY(by_shape(P, [&](auto sh) {
+        hipLaunchKernelGGL(kp_front_kernel<decltype(sh)>, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk,
+                           W.A, W.s, B);
+        return hipGetLastError();
+    }));
     TRY(product(P, backend, kMatVecT, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
     // b to the pk and to the sk's copy of it
     TRY(pack(W.v1, L * kN, pk, P.pk, 0, P.ep, L * kN, kXfNone, B, st, sk + P.indcpa_sk, P.sk));
@@ -748,9 +781,11 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
                       uint8_t *ct, uint8_t *ss, int64_t B, int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk, nullptr, W.buf, W.kr,
-                       W.A, W.s, W.v2, W.bits, B, (int64_t)0);
-    TRY(hipGetLastError());
+    TRY(by_shape(P, [&](auto sh) {
+        hipLaunchKernelGGL(enc_front_kernel<decltype(sh)>, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk,
+                           nullptr, W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, (int64_t)0);
+        return hipGetLastError();
+    }));
     TRY(enc_both_halves(P, W, backend, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
@@ -768,8 +803,11 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     const int64_t t0 = resident ? B : 0;
     TRY(hipEventRecord(W.fork, st));
     TRY(hipStreamWaitEvent(W.aux, W.fork, 0));
-    hipLaunchKernelGGL(dec_pre_kernel, dim3(pair_blocks(2 * B - t0)), dim3(64), 0, W.aux, P, ct, sk, W.A, W.hct, B, t0);
-    TRY(hipGetLastError());
+    TRY(by_shape(P, [&](auto sh) {
+        hipLaunchKernelGGL(dec_pre_kernel<decltype(sh)>, dim3(pair_blocks(2 * B - t0)), dim3(64), 0, W.aux, P, ct, sk,
+                           W.A, W.hct, B, t0);
+        return hipGetLastError();
+    }));
     TRY(hipEventRecord(W.join, W.aux));
     // IND-CPA decryption (S/src/cuda_kernel.cu:341-370; SB/src/SABLE_indcpa.cu:422-463)
     if (P.sable)
@@ -781,9 +819,11 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     // m' bits to W.bits, where the re-encryption's InnerProd reads them
     TRY(product(P, backend, kInner, kEpiDec, W.bits, W.v2, W.s, W.poly, B, status, st));
     // re-encryption of m' under the pk stored in the sk
-    hipLaunchKernelGGL(dec_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.bits, sk, W.buf, W.kr, W.s, W.v2,
-                       W.diff, B);
-    TRY(hipGetLastError());
+    TRY(by_shape(P, [&](auto sh) {
+        hipLaunchKernelGGL(dec_front_kernel<decltype(sh)>, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.bits, sk, W.buf,
+                           W.kr, W.s, W.v2, W.diff, B);
+        return hipGetLastError();
+    }));
     // ct is only read: the compare-packs never write through it
     uint8_t *ctc = const_cast<uint8_t *>(ct);
     TRY(enc_inner_half(P, W, backend, ctc, W.diff, B, status, st));
@@ -820,9 +860,11 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
     TRY(gather(reinterpret_cast<const uint8_t *>(K.A), abytes, keys, reinterpret_cast<uint8_t *>(W.A), abytes, abytes,
                B, st));
     TRY(gather(K.hpk, 32, keys, W.hpkg, 32, 32, B, st));
-    hipLaunchKernelGGL(enc_front_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg, W.buf, W.kr,
-                       W.A, W.s, W.v2, W.bits, B, B);
-    TRY(hipGetLastError());
+    TRY(by_shape(P, [&](auto sh) {
+        hipLaunchKernelGGL(enc_front_kernel<decltype(sh)>, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg,
+                           W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, B);
+        return hipGetLastError();
+    }));
     TRY(enc_both_halves(P, W, backend, ct, B, status, st));
     hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
     return hipGetLastError();
@@ -844,8 +886,11 @@ hipError_t kem_store_derive(const KemParams &P, const uint8_t *pk, int64_t n, ui
                             hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(matrix_kernel, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk + P.pvc, (int64_t)P.pk, A, n);
-    TRY(hipGetLastError());
+    TRY(by_shape(P, [&](auto sh) {
+        hipLaunchKernelGGL(matrix_kernel<decltype(sh)>, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk + P.pvc,
+                           (int64_t)P.pk, A, n);
+        return hipGetLastError();
+    }));
     hipLaunchKernelGGL(hash_pk_kernel, dim3(pair_blocks(n)), dim3(64), 0, st, P, pk, hpk, n);
     return hipGetLastError();
 }
