@@ -186,6 +186,54 @@ __device__ __forceinline__ void absorb(uint32_t (&A)[25], const uint8_t *in, int
     A[RW - 1] ^= p ? 0x80000000u : 0u;
 }
 
+// absorb with the next block's words loaded before the current block is
+// permuted, so their latency hides behind the permutation (17 more VGPRs).
+template <int RATE>
+__device__ __forceinline__ void absorb_prefetch(uint32_t (&A)[25], const uint8_t *in, int len, uint8_t ds, int p)
+{
+    constexpr int RW = RATE / 8;
+#pragma unroll
+    for (int i = 0; i < 25; i++) A[i] = 0;
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(in) + p;
+    uint32_t nxt[RW];
+    if (len >= RATE) {
+#pragma unroll
+        for (int i = 0; i < RW; i++) nxt[i] = w[2 * i];
+    }
+    while (len >= RATE) {
+#pragma unroll
+        for (int i = 0; i < RW; i++) A[i] ^= nxt[i];
+        w += 2 * RW;
+        len -= RATE;
+        if (len >= RATE) {
+#pragma unroll
+            for (int i = 0; i < RW; i++) nxt[i] = w[2 * i];
+        }
+        f1600(A, p);
+    }
+    const int full = len >> 3, rem = len & 7;
+    uint64_t last = (uint64_t)ds << (8 * rem);
+    const uint8_t *tail = reinterpret_cast<const uint8_t *>(w - p + 2 * full);
+    for (int b = 0; b < rem; b++) last |= (uint64_t)tail[b] << (8 * b);
+    const uint32_t lh = p ? (uint32_t)(last >> 32) : (uint32_t)last;
+#pragma unroll
+    for (int i = 0; i < RW; i++) {
+        if (i < full) A[i] ^= w[2 * i];
+        else if (i == full) A[i] ^= lh;
+    }
+    A[RW - 1] ^= p ? 0x80000000u : 0u;
+}
+
+// SHA3-256 through absorb_prefetch.
+__device__ __forceinline__ void sha3_256_prefetch(uint32_t (&out)[4], const uint8_t *in, int len, int p)
+{
+    uint32_t A[25];
+    absorb_prefetch<kSha3_256Rate>(A, in, len, kSha3DS, p);
+    f1600(A, p);
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[i] = A[i];
+}
+
 // Absorb N words held in registers (N * 8 bytes < rate).
 template <int N>
 __device__ __forceinline__ void absorb_words(uint32_t (&A)[25], const uint32_t (&m)[N], int rate, uint8_t ds, int p)

@@ -550,7 +550,7 @@ __global__ __launch_bounds__(64) void kp_hash_kernel(KemParams P, const uint8_t 
     TMVP_PAIR(0);
     if (t >= B) return;
     uint32_t h[4];
-    sha3_256(h, pk + t * P.pk, P.pk, p);
+    sha3_256_prefetch(h, pk + t * P.pk, P.pk, p);
     store_words(sk + t * P.sk + P.sk - 64, h, p);
 }
 
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(64) void enc_hash2_kernel(KemParams P, const uint8_
     TMVP_PAIR(0);
     if (t >= B) return;
     uint32_t h[4], k[8], s[4], k0[4];
-    sha3_256(h, ct + t * P.ct, P.ct, p);
+    sha3_256_prefetch(h, ct + t * P.ct, P.ct, p);
     load_words(k0, kr + t * 64, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
