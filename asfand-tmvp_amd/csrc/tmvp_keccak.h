@@ -108,13 +108,16 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
     return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
 
-__device__ __forceinline__ void f1600(uint32_t (&A)[25], int p)
+// U rounds per loop iteration (1 or 2).  At one or two sponge waves per SIMD,
+// 2 makes a lone permutation 6-11 % faster, but whole kernels gain or lose
+// with it depending on the code around it, so each kernel chooses.
+template <int U = 1> __device__ __forceinline__ void f1600(uint32_t (&A)[25], int p)
 {
     // The round constant is read at a uniform index (a scalar load) and the
     // lane's half picked with a mask: a per-lane index would make it a vector
     // load, and its wait would also wait for every store still in flight.
     const uint32_t hi_mask = 0u - (uint32_t)p;
-#pragma unroll 1
+#pragma unroll U
     for (int round = 0; round < 24; round++) {
         uint32_t C[5], R[5], B[25];
         // theta: column parities C, R = rotl(C, 1), A ^= C[x-1] ^ R[x+1]
@@ -159,7 +162,7 @@ template <int N> __device__ __forceinline__ void store_words(uint8_t *dst, const
 // Absorb `len` bytes (8-byte aligned source) with rate RATE and domain byte
 // `ds`, leaving the state ready for the first squeeze permutation.  Every
 // state index is a compile-time constant so the state stays in registers.
-template <int RATE>
+template <int RATE, int U = 1>
 __device__ __forceinline__ void absorb(uint32_t (&A)[25], const uint8_t *in, int len, uint8_t ds, int p)
 {
     constexpr int RW = RATE / 8;
@@ -169,7 +172,7 @@ __device__ __forceinline__ void absorb(uint32_t (&A)[25], const uint8_t *in, int
     while (len >= RATE) {
 #pragma unroll
         for (int i = 0; i < RW; i++) A[i] ^= w[2 * i];
-        f1600(A, p);
+        f1600<U>(A, p);
         w += 2 * RW;
         len -= RATE;
     }
@@ -188,7 +191,7 @@ __device__ __forceinline__ void absorb(uint32_t (&A)[25], const uint8_t *in, int
 
 // absorb with the next block's words loaded before the current block is
 // permuted, so their latency hides behind the permutation (17 more VGPRs).
-template <int RATE>
+template <int RATE, int U = 1>
 __device__ __forceinline__ void absorb_prefetch(uint32_t (&A)[25], const uint8_t *in, int len, uint8_t ds, int p)
 {
     constexpr int RW = RATE / 8;
@@ -209,7 +212,7 @@ __device__ __forceinline__ void absorb_prefetch(uint32_t (&A)[25], const uint8_t
 #pragma unroll
             for (int i = 0; i < RW; i++) nxt[i] = w[2 * i];
         }
-        f1600(A, p);
+        f1600<U>(A, p);
     }
     const int full = len >> 3, rem = len & 7;
     uint64_t last = (uint64_t)ds << (8 * rem);
@@ -225,11 +228,12 @@ __device__ __forceinline__ void absorb_prefetch(uint32_t (&A)[25], const uint8_t
 }
 
 // SHA3-256 through absorb_prefetch.
+template <int U = 1>
 __device__ __forceinline__ void sha3_256_prefetch(uint32_t (&out)[4], const uint8_t *in, int len, int p)
 {
     uint32_t A[25];
-    absorb_prefetch<kSha3_256Rate>(A, in, len, kSha3DS, p);
-    f1600(A, p);
+    absorb_prefetch<kSha3_256Rate, U>(A, in, len, kSha3DS, p);
+    f1600<U>(A, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = A[i];
 }
@@ -249,39 +253,39 @@ __device__ __forceinline__ void absorb_words(uint32_t (&A)[25], const uint32_t (
 }
 
 // SHA3-256 of an 8-byte-aligned buffer -> four words.
-__device__ __forceinline__ void sha3_256(uint32_t (&out)[4], const uint8_t *in, int len, int p)
+template <int U = 1> __device__ __forceinline__ void sha3_256(uint32_t (&out)[4], const uint8_t *in, int len, int p)
 {
     uint32_t A[25];
-    absorb<kSha3_256Rate>(A, in, len, kSha3DS, p);
-    f1600(A, p);
+    absorb<kSha3_256Rate, U>(A, in, len, kSha3DS, p);
+    f1600<U>(A, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = A[i];
 }
 
-template <int N> __device__ __forceinline__ void sha3_256_words(uint32_t (&out)[4], const uint32_t (&m)[N], int p)
+template <int N, int U = 1> __device__ __forceinline__ void sha3_256_words(uint32_t (&out)[4], const uint32_t (&m)[N], int p)
 {
     uint32_t A[25];
     absorb_words(A, m, kSha3_256Rate, kSha3DS, p);
-    f1600(A, p);
+    f1600<U>(A, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = A[i];
 }
 
-__device__ __forceinline__ void sha3_512_words(uint32_t (&out)[8], const uint32_t (&m)[8], int p)
+template <int U = 1> __device__ __forceinline__ void sha3_512_words(uint32_t (&out)[8], const uint32_t (&m)[8], int p)
 {
     uint32_t A[25];
     absorb_words(A, m, kSha3_512Rate, kSha3DS, p);
-    f1600(A, p);
+    f1600<U>(A, p);
 #pragma unroll
     for (int i = 0; i < 8; i++) out[i] = A[i];
 }
 
 // Squeeze `nwords` 64-bit words of SHAKE-128 output (absorbed, not yet
 // permuted state), calling f(k, half) with this lane's half of word k.
-template <class F> __device__ __forceinline__ void squeeze_each(uint32_t (&A)[25], int nwords, int p, F f)
+template <int U = 1, class F> __device__ __forceinline__ void squeeze_each(uint32_t (&A)[25], int nwords, int p, F f)
 {
     for (int k = 0; k < nwords; k += kShake128Rate / 8) {
-        f1600(A, p);
+        f1600<U>(A, p);
 #pragma unroll
         for (int i = 0; i < kShake128Rate / 8; i++)
             if (k + i < nwords) f(k + i, A[i]);
@@ -289,10 +293,10 @@ template <class F> __device__ __forceinline__ void squeeze_each(uint32_t (&A)[25
 }
 
 // Squeeze `nwords` words to dst, each lane writing its halves.
-__device__ __forceinline__ void squeeze_words(uint32_t (&A)[25], uint8_t *dst, int nwords, int p)
+template <int U = 1> __device__ __forceinline__ void squeeze_words(uint32_t (&A)[25], uint8_t *dst, int nwords, int p)
 {
     uint32_t *d = reinterpret_cast<uint32_t *>(dst) + p;
-    squeeze_each(A, nwords, p, [&](int k, uint32_t h) { d[2 * k] = h; });
+    squeeze_each<U>(A, nwords, p, [&](int k, uint32_t h) { d[2 * k] = h; });
 }
 
 } // namespace keccak

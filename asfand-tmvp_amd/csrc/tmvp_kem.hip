@@ -35,6 +35,12 @@ using namespace keccak;
 // sponge kernel works on pair t = g >> 1, half p = g & 1, and both lanes of a
 // pair always take the same branch.
 
+// Each sponge kernel picks the Keccak round unroll U (tmvp_keccak.h) that
+// measured faster for it at B = 16384: 2 for the encapsulation front,
+// key-store GenMatrix and the hash kernels; 1 for the key-generation front and
+// both decapsulation sponges (dec_pre shares the SIMDs with the decryption
+// chain, and its larger unrolled body slowed that chain more than it gained).
+
 // The compile-time shape of a parameter set's sampling: EQ (matrix bits), MU
 // (CBD width) and EP (bits of b).  Each sponge kernel is instantiated per
 // shape, so it holds only its own set's code paths (a kernel holding every
@@ -127,7 +133,7 @@ __device__ __forceinline__ void message_bits(const uint32_t (&m)[4], uint16_t *d
 // handed over at once.  A block of 21 words leaves at most G - 1 words of an
 // unfinished group behind, so RG groups of at least 21 + G - 1 words always
 // hold what is still needed.
-template <int G, int RG, class F>
+template <int G, int RG, int U, class F>
 __device__ __forceinline__ void stream_groups(const uint32_t (&seed)[4], int nwords, uint32_t *ring, int p, F f)
 {
     constexpr int RW = RG * G; // ring words
@@ -138,7 +144,7 @@ __device__ __forceinline__ void stream_groups(const uint32_t (&seed)[4], int nwo
     int g = 0;  // next group to hand over
     int kb = 0; // ring word of the block's first word
     for (int k = 0; k < nwords; k += RB) {
-        f1600(st, p);
+        f1600<U>(st, p);
 #pragma unroll
         for (int i = 0; i < RB; i++) {
             if (k + i < nwords) {
@@ -165,20 +171,20 @@ __device__ __forceinline__ void stream_groups(const uint32_t (&seed)[4], int nwo
 }
 
 // GenMatrix: 64 coefficients of EQ bits are exactly EQ words of the stream.
-template <int EQ>
+template <int EQ, int U>
 __device__ __forceinline__ void matrix_to_a(const uint32_t (&seed)[4], uint16_t *arow, int nwords, uint32_t *ring,
                                             int p)
 {
-    stream_groups<EQ, 3>(seed, nwords, ring, p,
+    stream_groups<EQ, 3, U>(seed, nwords, ring, p,
                          [&](int g, const uint32_t(&d)[EQ]) { unpack32<EQ>(d, arow + 64 * g + 32 * p); });
 }
 
 // A[t] from the 32-byte seed; `ring` is the instance's kRingDwords in LDS.
-template <int EQ>
+template <int EQ, int U>
 __device__ __forceinline__ void gen_matrix(const KemParams &P, const uint32_t (&seed)[4], uint16_t *arow,
                                            uint32_t *ring, int p)
 {
-    matrix_to_a<EQ>(seed, arow, P.araw / 8, ring, p);
+    matrix_to_a<EQ, U>(seed, arow, P.araw / 8, ring, p);
 }
 
 #define TMVP_RING                                                                                                    \
@@ -239,7 +245,7 @@ template <int MU> __device__ __forceinline__ void cbd32(const uint32_t (&d)[MU],
 }
 
 // The secret of one instance from its 32-byte seed, sampled into S.
-template <int MU>
+template <int MU, int U>
 __device__ __forceinline__ void secret_stream(const KemParams &P, const uint32_t (&seed)[4], uint16_t *S,
                                               uint32_t *ring, int p)
 {
@@ -248,12 +254,12 @@ __device__ __forceinline__ void secret_stream(const KemParams &P, const uint32_t
     if constexpr (MU == 8 || MU == 2) {
         uint32_t A[25];
         absorb_words(A, seed, kShake128Rate, kShakeDS, p);
-        if constexpr (MU == 8) squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd8_half(S, k, h, p, qmask); });
-        else squeeze_each(A, nwords, p, [&](int k, uint32_t h) { cbd2_half(S, k, h, p, qmask); });
+        if constexpr (MU == 8) squeeze_each<U>(A, nwords, p, [&](int k, uint32_t h) { cbd8_half(S, k, h, p, qmask); });
+        else squeeze_each<U>(A, nwords, p, [&](int k, uint32_t h) { cbd2_half(S, k, h, p, qmask); });
     } else {
         static_assert(MU == 6 || MU == 10, "CBD width");
         constexpr int RG = MU == 6 ? 5 : 3; // groups in the ring (at least 21 + MU - 1 words)
-        stream_groups<MU, RG>(seed, nwords, ring, p,
+        stream_groups<MU, RG, U>(seed, nwords, ring, p,
                               [&](int g, const uint32_t(&d)[MU]) { cbd32<MU>(d, S + 64 * g + 32 * p, qmask); });
     }
 }
@@ -261,6 +267,7 @@ __device__ __forceinline__ void secret_stream(const KemParams &P, const uint32_t
 // G(m | H(pk)) -> buf = m | H(pk), kr; returns the coins kr[32:64]; the
 // shared first step of encapsulation and of the re-encryption in
 // decapsulation (S/src/cuda_kernel.cu:116-118, :378-379).
+template <int U>
 __device__ __forceinline__ void hash_g(const uint32_t (&m)[4], const uint32_t (&hpk)[4], uint8_t *buf, uint8_t *kr,
                                        uint32_t (&coins)[4], int p)
 {
@@ -270,7 +277,7 @@ __device__ __forceinline__ void hash_g(const uint32_t (&m)[4], const uint32_t (&
         b[i] = m[i];
         b[4 + i] = hpk[i];
     }
-    sha3_512_words(k, b, p);
+    sha3_512_words<U>(k, b, p);
     store_words(buf, b, p);
     store_words(kr, k, p);
 #pragma unroll
@@ -287,27 +294,27 @@ __device__ __forceinline__ void hash_g(const uint32_t (&m)[4], const uint32_t (&
 // Key generation: seeds[t] = seed_A | seed_s | z.  Matrix role: seed_A' =
 // SHAKE-128(seed_A) goes to the pk tail (and the sk's copy of it) and seeds
 // GenMatrix.  Secret role: seed_s seeds the secret; z goes to the sk tail.
-template <class SH>
+template <class SH, int U>
 __device__ __forceinline__ void kp_matrix_role(const KemParams &P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
                                                uint16_t *A, int64_t t, uint32_t *ring, int p)
 {
     uint32_t m[4], st[25];
     load_words(m, seeds + t * 96, p);
     absorb_words(st, m, kShake128Rate, kShakeDS, p);
-    f1600(st, p);
+    f1600<U>(st, p);
     const uint32_t sa[4] = {st[0], st[1], st[2], st[3]};
     store_words(pk + t * P.pk + P.pvc, sa, p);
     store_words(sk + t * P.sk + P.indcpa_sk + P.pvc, sa, p);
-    gen_matrix<SH::EQ>(P, sa, A + t * P.l * P.l * kN, ring, p);
+    gen_matrix<SH::EQ, U>(P, sa, A + t * P.l * P.l * kN, ring, p);
 }
 
-template <class SH>
+template <class SH, int U>
 __device__ __forceinline__ void kp_secret_role(const KemParams &P, const uint8_t *seeds, uint8_t *sk, uint16_t *S,
                                                int64_t t, uint32_t *ring, int p)
 {
     uint32_t m[4];
     load_words(m, seeds + t * 96 + 32, p);
-    secret_stream<SH::MU>(P, m, S + t * P.l * kN, ring, p);
+    secret_stream<SH::MU, U>(P, m, S + t * P.l * kN, ring, p);
     load_words(m, seeds + t * 96 + 64, p);
     store_words(sk + t * P.sk + P.sk - 32, m, p);
 }
@@ -316,27 +323,27 @@ __device__ __forceinline__ void kp_secret_role(const KemParams &P, const uint8_t
 // m = H(rnd), kr = G(m | H(pk)), the secret from kr[32:64], and the InnerProd
 // operands: b of the pk to V and m's bits to bits.  hpk (if not null) holds
 // H(pk) per instance.
-template <class SH>
+template <class SH, int U>
 __device__ __forceinline__ void enc_matrix_role(const KemParams &P, const uint8_t *pk, uint16_t *A, int64_t t,
                                                 uint32_t *ring, int p)
 {
     uint32_t m[4];
     load_words(m, pk + t * P.pk + P.pvc, p);
-    gen_matrix<SH::EQ>(P, m, A + t * P.l * P.l * kN, ring, p);
+    gen_matrix<SH::EQ, U>(P, m, A + t * P.l * P.l * kN, ring, p);
 }
 
-template <class SH>
+template <class SH, int U>
 __device__ __forceinline__ void enc_secret_role(const KemParams &P, const uint8_t *rnd, const uint8_t *pk,
                                                 const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *S,
                                                 uint16_t *V, uint16_t *bits, int64_t t, uint32_t *ring, int p)
 {
     uint32_t r[4], m[4], h[4], coins[4];
     load_words(r, rnd + t * 32, p);
-    sha3_256_words(m, r, p);
+    sha3_256_words<4, U>(m, r, p);
     if (hpk) load_words(h, hpk + t * 32, p);
-    else sha3_256(h, pk + t * P.pk, P.pk, p);
-    hash_g(m, h, buf + t * 64, kr + t * 64, coins, p);
-    secret_stream<SH::MU>(P, coins, S + t * P.l * kN, ring, p);
+    else sha3_256<U>(h, pk + t * P.pk, P.pk, p);
+    hash_g<U>(m, h, buf + t * 64, kr + t * 64, coins, p);
+    secret_stream<SH::MU, U>(P, coins, S + t * P.l * kN, ring, p);
     message_bits(m, bits + t * kN, p);
     unpack_b<SH::EP>(P, pk + t * P.pk, V + t * P.l * kN, p);
 }
@@ -348,8 +355,8 @@ __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t
     TMVP_PAIR(0);
     TMVP_RING;
     if (t >= 2 * B) return;
-    if (t < B) kp_matrix_role<SH>(P, seeds, pk, sk, A, t, ring, p);
-    else kp_secret_role<SH>(P, seeds, sk, S, t - B, ring, p);
+    if (t < B) kp_matrix_role<SH, 1>(P, seeds, pk, sk, A, t, ring, p);
+    else kp_secret_role<SH, 1>(P, seeds, sk, S, t - B, ring, p);
 }
 
 template <class SH>
@@ -361,8 +368,8 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
     TMVP_PAIR(t0);
     TMVP_RING;
     if (t >= 2 * B) return;
-    if (t < B) enc_matrix_role<SH>(P, pk, A, t, ring, p);
-    else enc_secret_role<SH>(P, rnd, pk, hpk, buf, kr, S, V, bits, t - B, ring, p);
+    if (t < B) enc_matrix_role<SH, 2>(P, pk, A, t, ring, p);
+    else enc_secret_role<SH, 2>(P, rnd, pk, hpk, buf, kr, S, V, bits, t - B, ring, p);
 }
 
 // Decapsulation, before the decryption and beside it (on the auxiliary
@@ -378,11 +385,11 @@ __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t 
     if (t < B) {
         uint32_t m[4];
         load_words(m, sk + t * P.sk + P.indcpa_sk + P.pvc, p);
-        gen_matrix<SH::EQ>(P, m, A + t * P.l * P.l * kN, ring, p);
+        gen_matrix<SH::EQ, 1>(P, m, A + t * P.l * P.l * kN, ring, p);
     } else {
         const int64_t i = t - B;
         uint32_t h[4];
-        sha3_256(h, ct + i * P.ct, P.ct, p);
+        sha3_256<1>(h, ct + i * P.ct, P.ct, p);
         store_words(hct + i * 32, h, p);
     }
 }
@@ -414,8 +421,8 @@ __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16
     }
     uint32_t h[4], coins[4];
     load_words(h, sk + t * P.sk + P.sk - 64, p);
-    hash_g(m, h, buf + t * 64, kr + t * 64, coins, p);
-    secret_stream<SH::MU>(P, coins, S + t * P.l * kN, ring, p);
+    hash_g<1>(m, h, buf + t * 64, kr + t * 64, coins, p);
+    secret_stream<SH::MU, 1>(P, coins, S + t * P.l * kN, ring, p);
     unpack_b<SH::EP>(P, sk + t * P.sk + P.indcpa_sk, V + t * P.l * kN, p);
 }
 
@@ -527,7 +534,7 @@ __global__ __launch_bounds__(64) void hash_pk_kernel(KemParams P, const uint8_t 
     TMVP_PAIR(0);
     if (t >= B) return;
     uint32_t h[4];
-    sha3_256(h, pk + t * P.pk, P.pk, p);
+    sha3_256<2>(h, pk + t * P.pk, P.pk, p);
     store_words(hpk + t * 32, h, p);
 }
 
@@ -541,7 +548,7 @@ __global__ __launch_bounds__(64) void matrix_kernel(KemParams P, const uint8_t *
     if (t >= B) return;
     uint32_t m[4];
     load_words(m, seed + t * stride, p);
-    gen_matrix<SH::EQ>(P, m, A + t * P.l * P.l * kN, ring, p);
+    gen_matrix<SH::EQ, 2>(P, m, A + t * P.l * P.l * kN, ring, p);
 }
 
 // Key generation, last step: H(pk) into the sk.
@@ -550,7 +557,7 @@ __global__ __launch_bounds__(64) void kp_hash_kernel(KemParams P, const uint8_t 
     TMVP_PAIR(0);
     if (t >= B) return;
     uint32_t h[4];
-    sha3_256_prefetch(h, pk + t * P.pk, P.pk, p);
+    sha3_256_prefetch<2>(h, pk + t * P.pk, P.pk, p);
     store_words(sk + t * P.sk + P.sk - 64, h, p);
 }
 
@@ -561,14 +568,14 @@ __global__ __launch_bounds__(64) void enc_hash2_kernel(KemParams P, const uint8_
     TMVP_PAIR(0);
     if (t >= B) return;
     uint32_t h[4], k[8], s[4], k0[4];
-    sha3_256_prefetch(h, ct + t * P.ct, P.ct, p);
+    sha3_256_prefetch<2>(h, ct + t * P.ct, P.ct, p);
     load_words(k0, kr + t * 64, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         k[i] = k0[i];
         k[4 + i] = h[i];
     }
-    sha3_256_words(s, k, p);
+    sha3_256_words<8, 2>(s, k, p);
     store_words(ss + t * 32, s, p);
 }
 
@@ -592,7 +599,7 @@ __global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint32_t
         k[i] = k0[i] ^ (mask & (k0[i] ^ z[i]));
         k[4 + i] = h[i];
     }
-    sha3_256_words(s, k, p);
+    sha3_256_words<8, 1>(s, k, p);
     store_words(ss + t * 32, s, p);
     if (fail && p == 0) fail[t] = (uint8_t)(mask & 1u);
 }
