@@ -17,6 +17,12 @@
 //    arrival order up to max_batch instances, and retires finished slots by
 //    polling their events.  Requests arriving meanwhile join the next batch
 //    that a free slot takes.
+//  - Inputs reach the device through one copy-in kernel per batch that reads
+//    page-locked host memory directly: the key ids and pageable inputs from
+//    the slot's pinned staging, pinned inputs from the caller's own buffer.
+//    Submitting host-to-device DMA copies instead occasionally blocked the
+//    submitting thread for 7-11 ms inside the runtime, which stalled every
+//    slot.  Results return by DMA (device-to-host copies never blocked).
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -54,17 +60,59 @@ struct Pending {
     uint64_t id;
     tmvp_srv_request req;
     double t_submit;
-    bool pinned_in, pinned_out; // host buffers the DMA engines can reach directly
+    bool pinned_in, pinned_out; // host buffers the device can reach directly
+    const uint8_t *dev_in;      // pinned_in: the device's address of req.in
 };
 
-// Whether p lies in page-locked host memory from hipHostMalloc.
-bool is_pinned(const void *p)
+// Whether p lies in page-locked host memory from hipHostMalloc; *dev gets
+// the device's address of p when it does.
+bool is_pinned(const void *p, const void **dev = nullptr)
 {
     if (!p) return true;
     hipPointerAttribute_t a;
     const hipError_t e = hipPointerGetAttributes(&a, p);
     (void)hipGetLastError(); // pageable memory reports an error: clear it
-    return e == hipSuccess && a.type == hipMemoryTypeHost;
+    const bool ok = e == hipSuccess && a.type == hipMemoryTypeHost && a.devicePointer;
+    if (ok && dev) *dev = a.devicePointer;
+    return ok;
+}
+
+// One piece of a batch's input: `bytes` from src (page-locked host memory,
+// read over the bus) to dst (device memory).
+struct CopyDesc {
+    const uint8_t *src;
+    uint8_t *dst;
+    uint32_t bytes;
+};
+constexpr uint32_t kPiece = 64u << 10; // bytes per descriptor: one workgroup each
+
+// Workgroup b copies piece b.  16-byte accesses when both ends and the size
+// allow, else 4-byte, else bytes; each thread keeps four loads in flight.
+__global__ __launch_bounds__(256) void copy_in_kernel(const CopyDesc *descs, int n)
+{
+    if ((int)blockIdx.x >= n) return;
+    const CopyDesc d = descs[blockIdx.x];
+    const uintptr_t al = (uintptr_t)d.src | (uintptr_t)d.dst | d.bytes;
+    if ((al & 15) == 0) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(d.src);
+        uint4 *dst = reinterpret_cast<uint4 *>(d.dst);
+        const uint32_t m = d.bytes / 16;
+        for (uint32_t i = threadIdx.x; i < m; i += 4 * 256) {
+            uint4 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (i + k * 256 < m) v[k] = src[i + k * 256];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (i + k * 256 < m) dst[i + k * 256] = v[k];
+        }
+    } else if ((al & 3) == 0) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(d.src);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(d.dst);
+        for (uint32_t i = threadIdx.x; i < d.bytes / 4; i += 256) dst[i] = src[i];
+    } else {
+        for (uint32_t i = threadIdx.x; i < d.bytes; i += 256) d.dst[i] = d.src[i];
+    }
 }
 
 struct Slot {
@@ -77,6 +125,9 @@ struct Slot {
     uint8_t *d_out = nullptr; // ct | ss | fail
     int *d_status = nullptr;
     uint8_t *h_in = nullptr, *h_out = nullptr; // pinned staging
+    const uint8_t *h_in_dev = nullptr;         // the device's address of h_in
+    CopyDesc *h_desc = nullptr;                // pinned: the copy-in pieces
+    const CopyDesc *h_desc_dev = nullptr;
     int *h_status = nullptr;
     bool busy = false;
     int op = 0, ps = 0;
@@ -92,6 +143,7 @@ struct tmvp_server {
     int device, backend;
     int64_t max_batch;
     size_t ws_bytes, in_bytes, out_bytes;
+    int64_t desc_cap; // copy-in pieces one batch can need
     Clock::time_point t0;
     Store store[TMVP_NUM_PARAM_SETS];
     std::vector<Slot> slots;
@@ -145,6 +197,7 @@ void free_slot(Slot &s)
     if (s.h_in) (void)hipHostFree(s.h_in);
     if (s.h_out) (void)hipHostFree(s.h_out);
     if (s.h_status) (void)hipHostFree(s.h_status);
+    if (s.h_desc) (void)hipHostFree(s.h_desc);
     s = Slot();
 }
 
@@ -161,8 +214,14 @@ int alloc_slot(tmvp_server *S, Slot &s)
         return TMVP_ENOMEM;
     if (hipHostMalloc(&s.h_in, S->in_bytes, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&s.h_out, S->out_bytes, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(&s.h_status, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc(&s.h_status, sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&s.h_desc, (size_t)S->desc_cap * sizeof(CopyDesc), hipHostMallocDefault) != hipSuccess)
         return TMVP_ENOMEM;
+    const void *dv = nullptr;
+    if (!is_pinned(s.h_in, &dv)) return TMVP_EHIP;
+    s.h_in_dev = static_cast<const uint8_t *>(dv);
+    if (!is_pinned(s.h_desc, &dv)) return TMVP_EHIP;
+    s.h_desc_dev = static_cast<const CopyDesc *>(dv);
     if (hipMemset(s.d_status, 0, sizeof(int)) != hipSuccess) return TMVP_EHIP;
     return TMVP_OK;
 }
@@ -172,35 +231,42 @@ int launch_slot(tmvp_server *S, Slot &s)
 {
     const KemParams P = kem_params_of(s.ps);
     const int64_t B = s.instances;
-    // keys through the pinned staging; inputs by DMA from the request's own
-    // buffer when it is pinned, else through the staging
+    // Key ids, and the inputs of requests in pageable memory, go through the
+    // pinned staging; one copy-in kernel then reads them, and the pinned
+    // requests' inputs in place, into d_in (keys | inputs).
     int32_t *hkeys = reinterpret_cast<int32_t *>(s.h_in);
-    uint8_t *hin = s.h_in + up((size_t)B * 4);
-    uint8_t *din_base = s.d_in + up((size_t)B * 4);
+    const size_t in_off = up((size_t)B * 4);
+    uint8_t *din_base = s.d_in + in_off;
     const size_t in_each = s.op == TMVP_SRV_ENCAPS ? 32 : (size_t)P.ct;
-    int64_t off = 0, staged_lo = -1, staged_hi = -1;
+    int nd = 0;
+    auto add = [&](const uint8_t *src, uint8_t *dst, size_t bytes) {
+        for (size_t o = 0; o < bytes; o += kPiece) {
+            if (nd >= S->desc_cap) return false;
+            const size_t b = bytes - o < kPiece ? bytes - o : kPiece;
+            s.h_desc[nd++] = CopyDesc{src + o, dst + o, (uint32_t)b};
+        }
+        return true;
+    };
+    int64_t off = 0;
     for (const Pending &p : s.reqs) {
         memcpy(hkeys + off, p.req.keys, (size_t)p.req.count * 4);
         off += p.req.count;
     }
-    if (hipMemcpyAsync(s.d_in, s.h_in, (size_t)B * 4, hipMemcpyHostToDevice, s.stream) != hipSuccess) return TMVP_EHIP;
+    bool fits = add(s.h_in_dev, s.d_in, (size_t)B * 4);
     off = 0;
     for (const Pending &p : s.reqs) {
-        const size_t bytes = (size_t)p.req.count * in_each;
+        const size_t bytes = (size_t)p.req.count * in_each, at = (size_t)off * in_each;
         if (p.pinned_in) {
-            if (hipMemcpyAsync(din_base + off * in_each, p.req.in, bytes, hipMemcpyHostToDevice, s.stream) != hipSuccess)
-                return TMVP_EHIP;
+            fits = fits && add(p.dev_in, din_base + at, bytes);
         } else {
-            memcpy(hin + off * in_each, p.req.in, bytes);
-            if (staged_lo < 0) staged_lo = off;
-            staged_hi = off + p.req.count;
+            memcpy(s.h_in + in_off + at, p.req.in, bytes);
+            fits = fits && add(s.h_in_dev + in_off + at, din_base + at, bytes);
         }
         off += p.req.count;
     }
-    if (staged_lo >= 0 &&
-        hipMemcpyAsync(din_base + staged_lo * in_each, hin + staged_lo * in_each,
-                       (size_t)(staged_hi - staged_lo) * in_each, hipMemcpyHostToDevice, s.stream) != hipSuccess)
-        return TMVP_EHIP;
+    if (!fits) return TMVP_EARG; // cannot happen: desc_cap covers max_batch requests
+    hipLaunchKernelGGL(copy_in_kernel, dim3((unsigned)nd), dim3(256), 0, s.stream, s.h_desc_dev, nd);
+    if (hipGetLastError() != hipSuccess) return TMVP_EHIP;
     KemWorkspace W;
     kem_workspace_carve(P, B, s.ws, W);
     W.aux = s.aux;
@@ -456,6 +522,8 @@ int tmvp_srv_create(tmvp_ctx *ctx, const tmvp_srv_config *cfg, tmvp_server **out
     S->max_batch = c.max_batch;
     S->t0 = Clock::now();
     max_sizes(c.max_batch, S->ws_bytes, S->in_bytes, S->out_bytes);
+    // every request has at least one instance: pieces <= requests + bytes / kPiece, plus the keys
+    S->desc_cap = 2 * c.max_batch + (int64_t)(S->in_bytes / kPiece) + 4;
     const int64_t per_slot = (int64_t)(S->ws_bytes + S->in_bytes + S->out_bytes);
     int nslots = (int)(c.hbm_budget / per_slot);
     if (nslots > c.slots) nslots = c.slots;
@@ -519,7 +587,8 @@ int tmvp_srv_submit(tmvp_server *S, const tmvp_srv_request *r, uint64_t *id)
     if (r->param_set < 0 || r->param_set >= TMVP_NUM_PARAM_SETS) return TMVP_EARG;
     if (r->count < 1 || !r->keys || !r->in || !r->ss || (r->op == TMVP_SRV_ENCAPS && !r->ct)) return TMVP_EARG;
     // outside the lock: pointer queries of the caller's buffers
-    const bool pinned_in = is_pinned(r->in);
+    const void *dev_in = nullptr;
+    const bool pinned_in = is_pinned(r->in, &dev_in);
     const bool pinned_out = is_pinned(r->ss) && (r->op != TMVP_SRV_ENCAPS || is_pinned(r->ct)) &&
                             (r->op != TMVP_SRV_DECAPS || is_pinned(r->fail));
     std::lock_guard<std::mutex> g(S->mu);
@@ -534,6 +603,7 @@ int tmvp_srv_submit(tmvp_server *S, const tmvp_srv_request *r, uint64_t *id)
     p.t_submit = S->now_us();
     p.pinned_in = pinned_in;
     p.pinned_out = pinned_out;
+    p.dev_in = static_cast<const uint8_t *>(dev_in);
     S->queue[r->param_set][r->op].push_back(p);
     S->pending_requests += 1;
     S->stats.requests_submitted += 1;

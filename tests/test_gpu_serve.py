@@ -182,3 +182,38 @@ def test_serving_pinned_buffers(ctx, keys):
     for k, (op, ps, ids, data, want) in enumerate(reqs):
         for got, exp in zip(outs[k], want):
             assert np.array_equal(got, exp), (k, op, ps)
+
+
+def test_serving_pinned_between_pageable(ctx, keys):
+    """Within one batch, a request whose input is page-locked sits between two
+    whose inputs are pageable: each request's input reaches the device whole,
+    whichever memory it lives in."""
+    rng = np.random.default_rng(21)
+    reqs = make_requests(ctx, keys, rng, 12, 200)
+    srv = T.Server(ctx, max_batch=4096, slots=1)
+    try:
+        for ps in SETS:
+            srv.add_keys(ps, *keys[ps])
+        # requests k, k + 4, k + 8 share a (set, operation): pageable, pinned, pageable
+        inputs = []
+        for k, (op, ps, ids, data, want) in enumerate(reqs):
+            if (k // 4) % 2 == 1:
+                pdata = T.pinned_array(data.shape)
+                pdata[:] = data
+                inputs.append(pdata)
+            else:
+                inputs.append(data.copy())
+        outs = {}
+        for k, (op, ps, ids, data, want) in enumerate(reqs):
+            _, outs[k] = srv.submit(op, ps, ids, inputs[k], tag=k)
+        done = {}
+        while len(done) < len(reqs):
+            for c in srv.poll(64, timeout_us=200000):
+                done[c["tag"]] = c
+        assert all(c["status"] == 0 for c in done.values())
+        assert max(c["batch_requests"] for c in done.values()) >= 2
+    finally:
+        srv.close()
+    for k, (op, ps, ids, data, want) in enumerate(reqs):
+        for got, exp in zip(outs[k], want):
+            assert np.array_equal(got, exp), (k, op, ps)
