@@ -419,8 +419,9 @@ def main():
                     help="only run device-to-device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(instances are independent; see DESIGN.md for how B would shard).", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(instances are independent, so a batch would split across GPUs with no exchange step).",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.calib_copy_mib:
