@@ -264,6 +264,48 @@ __device__ __forceinline__ void secret_stream(const KemParams &P, const uint32_t
     }
 }
 
+// The secret of one instance packed into the sk (S/src/pack.cu:51-65 code,
+// SB/src/pack.cu:366-377 for Sable's 2-bit one): chunk c of 64 coefficients is
+// W 64-bit words of the LSB-first W-bit code, and lane p's 32 coefficients
+// 32p..32p+31 of it are exactly dwords pW..pW+W-1.  The sampling wrote S with
+// the pair's lanes interleaved, so each lane re-reads its half, ordered after
+// those stores by a workgroup-scope fence (the workgroup is this one wave; an
+// agent-scope fence would also write back the L2, which made key generation
+// 24 us slower).  SK2: the Sable code s & 3.
+template <int W, bool SK2>
+__device__ __forceinline__ void pack_secret_half(const uint16_t *S, uint8_t *dst, int nchunks, int p)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    constexpr uint32_t mask = SK2 ? 3u : (1u << W) - 1u;
+    for (int c = 0; c < nchunks; c++) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(S + 64 * c + 32 * p);
+        uint32_t e[32];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 v = src[q];
+            const uint32_t h[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                e[8 * q + 2 * k] = h[k] & mask;
+                e[8 * q + 2 * k + 1] = (h[k] >> 16) & mask;
+            }
+        }
+        uint32_t w[W];
+#pragma unroll
+        for (int i = 0; i < W; i++) w[i] = 0;
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+            const int bit = W * j, i = bit >> 5, o = bit & 31;
+            w[i] |= e[j] << o;
+            if (o + W > 32) w[i + 1] |= e[j] >> (32 - o);
+        }
+        uint32_t *d = reinterpret_cast<uint32_t *>(dst + (int64_t)c * W * 8) + p * W;
+#pragma unroll
+        for (int i = 0; i < W; i++) d[i] = w[i];
+    }
+}
+
 // G(m | H(pk)) -> buf = m | H(pk), kr; returns the coins kr[32:64]; the
 // shared first step of encapsulation and of the re-encryption in
 // decapsulation (S/src/cuda_kernel.cu:116-118, :378-379).
@@ -293,7 +335,8 @@ __device__ __forceinline__ void hash_g(const uint32_t (&m)[4], const uint32_t (&
 
 // Key generation: seeds[t] = seed_A | seed_s | z.  Matrix role: seed_A' =
 // SHAKE-128(seed_A) goes to the pk tail (and the sk's copy of it) and seeds
-// GenMatrix.  Secret role: seed_s seeds the secret; z goes to the sk tail.
+// GenMatrix.  Secret role: seed_s seeds the secret, which goes to S and,
+// packed, to the head of the sk; z goes to the sk tail.
 template <class SH, int U>
 __device__ __forceinline__ void kp_matrix_role(const KemParams &P, const uint8_t *seeds, uint8_t *pk, uint8_t *sk,
                                                uint16_t *A, int64_t t, uint32_t *ring, int p)
@@ -315,6 +358,8 @@ __device__ __forceinline__ void kp_secret_role(const KemParams &P, const uint8_t
     uint32_t m[4];
     load_words(m, seeds + t * 96 + 32, p);
     secret_stream<SH::MU, U>(P, m, S + t * P.l * kN, ring, p);
+    if constexpr (SH::MU == 2) pack_secret_half<2, true>(S + t * P.l * kN, sk + t * P.sk, 4 * P.l, p);
+    else pack_secret_half<SH::EQ, false>(S + t * P.l * kN, sk + t * P.sk, 4 * P.l, p);
     load_words(m, seeds + t * 96 + 64, p);
     store_words(sk + t * P.sk + P.sk - 32, m, p);
 }
@@ -774,12 +819,8 @@ hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, c
         return hipGetLastError();
     }));
     TRY(product(P, backend, kMatVecT, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
-    // b to the pk and to the sk's copy of it
+    // b to the pk and to the sk's copy of it (the front kernel packed s into the sk)
     TRY(pack(W.v1, L * kN, pk, P.pk, 0, P.ep, L * kN, kXfNone, B, st, sk + P.indcpa_sk, P.sk));
-    if (P.sable)
-        TRY(pack(W.s, L * kN, sk, P.sk, 0, 2, L * kN, kXfSk2, B, st));
-    else
-        TRY(pack(W.s, L * kN, sk, P.sk, 0, P.eq, L * kN, kXfNone, B, st));
     hipLaunchKernelGGL(kp_hash_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, pk, sk, B);
     return hipGetLastError();
 }
