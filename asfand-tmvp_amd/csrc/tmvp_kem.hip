@@ -14,11 +14,13 @@
 //
 // Sponges run on a pair of lanes per instance (state in registers, split in
 // 32-bit halves) and write their results in the form the products read: A
-// (through an LDS ring), s (CBD for mu = 8 and 2), b of the public key and the
-// message bits.  The other codecs, and CBD for mu = 6 and 10, run one thread
-// per 64 coefficients.
+// (through an LDS ring), s, b of the public key and the message bits.  The
+// lane pairs also pack the secret into the sk (key generation) and the
+// ciphertext before hashing it (encapsulation); the other codecs run one
+// thread per 64 coefficients.
 #include <string.h>
 
+#include <type_traits>
 #include <utility>
 
 #include "tmvp_internal.h"
@@ -264,19 +266,25 @@ __device__ __forceinline__ void secret_stream(const KemParams &P, const uint32_t
     }
 }
 
-// The secret of one instance packed into the sk (S/src/pack.cu:51-65 code,
-// SB/src/pack.cu:366-377 for Sable's 2-bit one): chunk c of 64 coefficients is
-// W 64-bit words of the LSB-first W-bit code, and lane p's 32 coefficients
-// 32p..32p+31 of it are exactly dwords pW..pW+W-1.  The sampling wrote S with
-// the pair's lanes interleaved, so each lane re-reads its half, ordered after
-// those stores by a workgroup-scope fence (the workgroup is this one wave; an
-// agent-scope fence would also write back the L2, which made key generation
-// 24 us slower).  SK2: the Sable code s & 3.
-template <int W, bool SK2>
-__device__ __forceinline__ void pack_secret_half(const uint16_t *S, uint8_t *dst, int nchunks, int p)
+// One instance's polynomials packed by its lane pair (the codecs of
+// S/src/pack.cu:1-213 and SB/src/pack.cu:182-467): chunk c of 64 coefficients
+// is W 64-bit words of the LSB-first W-bit code, and lane p's 32 coefficients
+// 32p..32p+31 of it are exactly dwords pW..pW+W-1.  SK2: the Sable secret
+// code s & 3 (SB/src/pack.cu:366-377).
+//
+// Callers order these accesses against the pair's other ones with a
+// workgroup-scope fence (the workgroup is this one wave): one after the stores
+// that wrote the source, one before loads of what was packed.  An agent-scope
+// fence would also write back the L2, which made key generation 24 us slower.
+__device__ __forceinline__ void wave_fence_global()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <int W, bool SK2>
+__device__ __forceinline__ void pack_half(const uint16_t *S, uint8_t *dst, int nchunks, int p)
+{
     constexpr uint32_t mask = SK2 ? 3u : (1u << W) - 1u;
     for (int c = 0; c < nchunks; c++) {
         const uint4 *src = reinterpret_cast<const uint4 *>(S + 64 * c + 32 * p);
@@ -358,8 +366,9 @@ __device__ __forceinline__ void kp_secret_role(const KemParams &P, const uint8_t
     uint32_t m[4];
     load_words(m, seeds + t * 96 + 32, p);
     secret_stream<SH::MU, U>(P, m, S + t * P.l * kN, ring, p);
-    if constexpr (SH::MU == 2) pack_secret_half<2, true>(S + t * P.l * kN, sk + t * P.sk, 4 * P.l, p);
-    else pack_secret_half<SH::EQ, false>(S + t * P.l * kN, sk + t * P.sk, 4 * P.l, p);
+    wave_fence_global(); // the sampling wrote S with the pair's lanes interleaved
+    if constexpr (SH::MU == 2) pack_half<2, true>(S + t * P.l * kN, sk + t * P.sk, 4 * P.l, p);
+    else pack_half<SH::EQ, false>(S + t * P.l * kN, sk + t * P.sk, 4 * P.l, p);
     load_words(m, seeds + t * 96 + 64, p);
     store_words(sk + t * P.sk + P.sk - 32, m, p);
 }
@@ -606,12 +615,20 @@ __global__ __launch_bounds__(64) void kp_hash_kernel(KemParams P, const uint8_t 
     store_words(sk + t * P.sk + P.sk - 64, h, p);
 }
 
-// Encapsulation, last step (S/src/cuda_kernel.cu:185-186): kr[32:] = H(ct), ss = H(kr).
-__global__ __launch_bounds__(64) void enc_hash2_kernel(KemParams P, const uint8_t *ct, uint8_t *kr, uint8_t *ss,
-                                                       int64_t B)
+// Encapsulation, last step: with EP > 0 the ciphertext is first packed from
+// b' (bp, EP bits) and v' (vp, ET bits) (S/src/cuda_kernel.cu:133, :178);
+// then kr[32:] = H(ct), ss = H(kr) (:185-186).
+template <int EP, int ET>
+__global__ __launch_bounds__(64) void enc_hash2_kernel(KemParams P, uint8_t *ct, uint8_t *kr, uint8_t *ss, int64_t B,
+                                                       const uint16_t *bp, const uint16_t *vp)
 {
     TMVP_PAIR(0);
     if (t >= B) return;
+    if constexpr (EP > 0) {
+        pack_half<EP, false>(bp + t * P.l * kN, ct + t * P.ct, 4 * P.l, p);
+        pack_half<ET, false>(vp + t * kN, ct + t * P.ct + P.pvc, 4, p);
+        wave_fence_global(); // H(ct) reads both lanes' words
+    }
     uint32_t h[4], k[8], s[4], k0[4];
     sha3_256_prefetch<2>(h, ct + t * P.ct, P.ct, p);
     load_words(k0, kr + t * 64, p);
@@ -787,21 +804,36 @@ static hipError_t enc_matvec_half(const KemParams &P, const KemWorkspace &W, int
 
 // The whole encryption.  In the MFMA form both products run as one: the L rows
 // of A and b as an extra row share every secret term's staging (kEnc).
-static hipError_t enc_both_halves(const KemParams &P, const KemWorkspace &W, int backend, uint8_t *ct, int64_t B,
-                                  int *status, hipStream_t st)
+// Encryption's products and the final hashes.  MFMA: A s' and b . s' as one
+// product, and the ciphertext packed inside the hash kernel; VALU: the two
+// halves, each packing its part, then the hash kernel.
+static hipError_t enc_finish(const KemParams &P, const KemWorkspace &W, int backend, uint8_t *ct, uint8_t *ss,
+                             int64_t B, int *status, hipStream_t st)
 {
+    const dim3 grid(pair_blocks(B));
     if (backend != 0) {
         TRY(enc_matvec_half(P, W, backend, ct, nullptr, B, status, st));
-        return enc_inner_half(P, W, backend, ct, nullptr, B, status, st);
+        TRY(enc_inner_half(P, W, backend, ct, nullptr, B, status, st));
+        hipLaunchKernelGGL((enc_hash2_kernel<0, 0>), grid, dim3(64), 0, st, P, ct, W.kr, ss, B, nullptr, nullptr);
+        return hipGetLastError();
     }
-    const int L = P.l;
     Job J = make_job(P, kMatVec, kEpiQ2P, W.v1, W.A, W.s, W.bits, B, status);
     J.b = W.v2;
     J.out2 = W.poly;
-    TRY(launch_mfma(J, kEnc, L, st));
-    TRY(pack(W.v1, L * kN, ct, P.ct, 0, P.ep, L * kN, kXfNone, B, st));
-    TRY(pack(W.poly, kN, ct, P.ct, P.pvc, P.et, kN, kXfNone, B, st));
-    return hipSuccess;
+    TRY(launch_mfma(J, kEnc, P.l, st));
+    auto go = [&](auto k) {
+        constexpr int EP = decltype(k)::first_type::value, ET = decltype(k)::second_type::value;
+        hipLaunchKernelGGL((enc_hash2_kernel<EP, ET>), grid, dim3(64), 0, st, P, ct, W.kr, ss, B, W.v1, W.poly);
+        return hipGetLastError();
+    };
+    using std::integral_constant;
+    using std::pair;
+    if (P.ep == 10 && P.et == 3) return go(pair<integral_constant<int, 10>, integral_constant<int, 3>>{});
+    if (P.ep == 10 && P.et == 4) return go(pair<integral_constant<int, 10>, integral_constant<int, 4>>{});
+    if (P.ep == 10 && P.et == 6) return go(pair<integral_constant<int, 10>, integral_constant<int, 6>>{});
+    if (P.ep == 9 && P.et == 3) return go(pair<integral_constant<int, 9>, integral_constant<int, 3>>{});
+    if (P.ep == 9 && P.et == 5) return go(pair<integral_constant<int, 9>, integral_constant<int, 5>>{});
+    return hipErrorInvalidValue;
 }
 
 // Key generation and encapsulation run on one stream: the front kernel keeps
@@ -834,9 +866,7 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
                            nullptr, W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, (int64_t)0);
         return hipGetLastError();
     }));
-    TRY(enc_both_halves(P, W, backend, ct, B, status, st));
-    hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
-    return hipGetLastError();
+    return enc_finish(P, W, backend, ct, ss, B, status, st);
 }
 
 // Decapsulation; with `resident` set, the matrix A of every instance is
@@ -913,9 +943,7 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
                            W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, B);
         return hipGetLastError();
     }));
-    TRY(enc_both_halves(P, W, backend, ct, B, status, st));
-    hipLaunchKernelGGL(enc_hash2_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, ct, W.kr, ss, B);
-    return hipGetLastError();
+    return enc_finish(P, W, backend, ct, ss, B, status, st);
 }
 
 hipError_t kem_decaps_keyed(const KemParams &P, const KemWorkspace &W, int backend, const KeyStore &K,
