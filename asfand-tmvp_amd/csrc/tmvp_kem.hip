@@ -286,31 +286,40 @@ template <int W, bool SK2>
 __device__ __forceinline__ void pack_half(const uint16_t *S, uint8_t *dst, int nchunks, int p)
 {
     constexpr uint32_t mask = SK2 ? 3u : (1u << W) - 1u;
-    for (int c = 0; c < nchunks; c++) {
-        const uint4 *src = reinterpret_cast<const uint4 *>(S + 64 * c + 32 * p);
-        uint32_t e[32];
+    constexpr int G = 4; // chunks whose loads are in flight together (nchunks is a multiple of 4)
+    for (int c0 = 0; c0 < nchunks; c0 += G) {
+        uint4 v[G][4];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint4 v = src[q];
-            const uint32_t h[4] = {v.x, v.y, v.z, v.w};
+        for (int g = 0; g < G; g++) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(S + 64 * (c0 + g) + 32 * p);
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                e[8 * q + 2 * k] = h[k] & mask;
-                e[8 * q + 2 * k + 1] = (h[k] >> 16) & mask;
+            for (int q = 0; q < 4; q++) v[g][q] = src[q];
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            uint32_t e[32];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t h[4] = {v[g][q].x, v[g][q].y, v[g][q].z, v[g][q].w};
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    e[8 * q + 2 * k] = h[k] & mask;
+                    e[8 * q + 2 * k + 1] = (h[k] >> 16) & mask;
+                }
             }
+            uint32_t w[W];
+#pragma unroll
+            for (int i = 0; i < W; i++) w[i] = 0;
+#pragma unroll
+            for (int j = 0; j < 32; j++) {
+                const int bit = W * j, i = bit >> 5, o = bit & 31;
+                w[i] |= e[j] << o;
+                if (o + W > 32) w[i + 1] |= e[j] >> (32 - o);
+            }
+            uint32_t *d = reinterpret_cast<uint32_t *>(dst + (int64_t)(c0 + g) * W * 8) + p * W;
+#pragma unroll
+            for (int i = 0; i < W; i++) d[i] = w[i];
         }
-        uint32_t w[W];
-#pragma unroll
-        for (int i = 0; i < W; i++) w[i] = 0;
-#pragma unroll
-        for (int j = 0; j < 32; j++) {
-            const int bit = W * j, i = bit >> 5, o = bit & 31;
-            w[i] |= e[j] << o;
-            if (o + W > 32) w[i + 1] |= e[j] >> (32 - o);
-        }
-        uint32_t *d = reinterpret_cast<uint32_t *>(dst + (int64_t)c * W * 8) + p * W;
-#pragma unroll
-        for (int i = 0; i < W; i++) d[i] = w[i];
     }
 }
 
