@@ -811,8 +811,6 @@ static hipError_t enc_matvec_half(const KemParams &P, const KemWorkspace &W, int
     return hipSuccess;
 }
 
-// The whole encryption.  In the MFMA form both products run as one: the L rows
-// of A and b as an extra row share every secret term's staging (kEnc).
 // Encryption's products and the final hashes.  MFMA: A s' and b . s' as one
 // product, and the ciphertext packed inside the hash kernel; VALU: the two
 // halves, each packing its part, then the hash kernel.
