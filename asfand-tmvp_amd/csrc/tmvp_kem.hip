@@ -283,7 +283,7 @@ __device__ __forceinline__ void wave_fence_global()
 }
 
 template <int W, bool SK2>
-__device__ __forceinline__ void pack_half(const uint16_t *S, uint8_t *dst, int nchunks, int p)
+__device__ __forceinline__ void pack_half(const uint16_t *S, uint8_t *dst, int nchunks, int p, uint8_t *dst2 = nullptr)
 {
     constexpr uint32_t mask = SK2 ? 3u : (1u << W) - 1u;
     constexpr int G = 4; // chunks whose loads are in flight together (nchunks is a multiple of 4)
@@ -319,6 +319,11 @@ __device__ __forceinline__ void pack_half(const uint16_t *S, uint8_t *dst, int n
             uint32_t *d = reinterpret_cast<uint32_t *>(dst + (int64_t)(c0 + g) * W * 8) + p * W;
 #pragma unroll
             for (int i = 0; i < W; i++) d[i] = w[i];
+            if (dst2) {
+                uint32_t *d2 = reinterpret_cast<uint32_t *>(dst2 + (int64_t)(c0 + g) * W * 8) + p * W;
+#pragma unroll
+                for (int i = 0; i < W; i++) d2[i] = w[i];
+            }
         }
     }
 }
@@ -615,10 +620,16 @@ __global__ __launch_bounds__(64) void matrix_kernel(KemParams P, const uint8_t *
 }
 
 // Key generation, last step: H(pk) into the sk.
-__global__ __launch_bounds__(64) void kp_hash_kernel(KemParams P, const uint8_t *pk, uint8_t *sk, int64_t B)
+// Key generation, last step: b (bp, EP bits) packed into the pk and the sk's
+// copy of it, then H(pk) into the sk.
+template <int EP>
+__global__ __launch_bounds__(64) void kp_hash_kernel(KemParams P, uint8_t *pk, uint8_t *sk, int64_t B,
+                                                     const uint16_t *bp)
 {
     TMVP_PAIR(0);
     if (t >= B) return;
+    pack_half<EP, false>(bp + t * P.l * kN, pk + t * P.pk, 4 * P.l, p, sk + t * P.sk + P.indcpa_sk);
+    wave_fence_global(); // H(pk) reads both lanes' words
     uint32_t h[4];
     sha3_256_prefetch<2>(h, pk + t * P.pk, P.pk, p);
     store_words(sk + t * P.sk + P.sk - 64, h, p);
@@ -851,16 +862,20 @@ hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, c
                        uint8_t *sk, int64_t B, int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
-    const int L = P.l;
     TRY(by_shape(P, [&](auto sh) {
         hipLaunchKernelGGL(kp_front_kernel<decltype(sh)>, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk,
                            W.A, W.s, B);
         return hipGetLastError();
     }));
     TRY(product(P, backend, kMatVecT, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
-    // b to the pk and to the sk's copy of it (the front kernel packed s into the sk)
-    TRY(pack(W.v1, L * kN, pk, P.pk, 0, P.ep, L * kN, kXfNone, B, st, sk + P.indcpa_sk, P.sk));
-    hipLaunchKernelGGL(kp_hash_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, pk, sk, B);
+    // the front kernel packed s into the sk; the hash kernel packs b into the
+    // pk and the sk's copy of it, then hashes the pk
+    if (P.ep == 10)
+        hipLaunchKernelGGL(kp_hash_kernel<10>, dim3(pair_blocks(B)), dim3(64), 0, st, P, pk, sk, B, W.v1);
+    else if (P.ep == 9)
+        hipLaunchKernelGGL(kp_hash_kernel<9>, dim3(pair_blocks(B)), dim3(64), 0, st, P, pk, sk, B, W.v1);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
