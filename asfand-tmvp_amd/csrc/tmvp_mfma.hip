@@ -3,6 +3,8 @@
 // layouts and the per-wave work (mfma_body) are in tmvp_mfma_body.h.
 #include <stdlib.h>
 
+#include <atomic>
+
 #include "tmvp_mfma_body.h"
 
 namespace tmvp {
@@ -22,7 +24,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEU
 template <int L, int MODE, int EPI>
 static hipError_t go(const Job &J, hipStream_t stream)
 {
-    static int resident = 0; // workgroups that fit on the device at once
+    // Workgroups that fit on the device at once; threads that race here
+    // compute the same value.
+    static std::atomic<int> resident_cache{0};
+    int resident = resident_cache.load(std::memory_order_relaxed);
     if (resident == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         hipError_t e = hipGetDevice(&dev);
@@ -36,6 +41,7 @@ static hipError_t go(const Job &J, hipStream_t stream)
             if (want >= 1 && want < per_cu) per_cu = want;
         }
         resident = cus * per_cu;
+        resident_cache.store(resident, std::memory_order_relaxed);
     }
     const int64_t needed = (J.batch + 3) / 4;
     const int64_t blocks = needed < resident ? needed : resident;
@@ -150,13 +156,13 @@ int mfma_kind_cost(int kind)
 
 int mfma_multi_per_cu()
 {
-    static int per_cu = 0;
-    if (per_cu == 0) {
-        int n = 0;
+    static std::atomic<int> per_cu{0}; // racing threads compute the same value
+    int n = per_cu.load(std::memory_order_relaxed);
+    if (n == 0) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, mfma_multi_kernel, 256, 0) != hipSuccess || n < 1) n = 1;
-        per_cu = n;
+        per_cu.store(n, std::memory_order_relaxed);
     }
-    return per_cu;
+    return n;
 }
 
 hipError_t launch_mfma_multi(const MultiJob &M, int grid, hipStream_t stream)
