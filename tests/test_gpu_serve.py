@@ -217,3 +217,39 @@ def test_serving_pinned_between_pageable(ctx, keys):
     for k, (op, ps, ids, data, want) in enumerate(reqs):
         for got, exp in zip(outs[k], want):
             assert np.array_equal(got, exp), (k, op, ps)
+
+
+@pytest.mark.parametrize("offset", [0, 4, 1])
+def test_serving_pinned_input_alignment(ctx, keys, offset):
+    """Page-locked inputs at 16-byte, 4-byte and odd addresses (the copy-in
+    kernel's three access widths) give the plain batched call's results."""
+    rng = np.random.default_rng(30 + offset)
+    srv = T.Server(ctx, max_batch=2048, slots=2)
+    try:
+        for ps in SETS:
+            srv.add_keys(ps, *keys[ps])
+        reqs = []
+        for ps in SETS:
+            n = 97
+            ids = rng.integers(0, NKEYS, n).astype(np.int32)
+            rnd = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+            ct, ss = T.run_kem_encaps(ctx, keys[ps][0][ids], rnd, ps, T.MFMA)
+            for op, data, want in ((T.SRV_ENCAPS, rnd, (ct, ss)),
+                                   (T.SRV_DECAPS, ct, T.run_kem_decaps(ctx, ct, keys[ps][1][ids], ps, T.MFMA))):
+                buf = T.pinned_array((data.size + 16,))
+                view = buf[offset:offset + data.size].reshape(data.shape)
+                view[:] = data
+                reqs.append((op, ps, ids, view, want))
+        outs = {}
+        for k, (op, ps, ids, view, _) in enumerate(reqs):
+            _, outs[k] = srv.submit(op, ps, ids, view, tag=k)
+        done = {}
+        while len(done) < len(reqs):
+            for c in srv.poll(64, timeout_us=200000):
+                done[c["tag"]] = c
+        assert all(c["status"] == 0 for c in done.values())
+    finally:
+        srv.close()
+    for k, (op, ps, ids, view, want) in enumerate(reqs):
+        for got, exp in zip(outs[k], want):
+            assert np.array_equal(got, exp), (k, op, ps, offset)
