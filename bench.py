@@ -452,6 +452,22 @@ def main():
 
     ms_mfma = time_launches(run_mfma, stream, args.steps, args.warmup)
     print(f"[bench] mfma {ms_mfma / args.steps * 1e3:.1f} us/launch", file=sys.stderr, flush=True)
+    # the same launch captured in a graph: median of 25 replays, each between
+    # its own pair of events (so each pays its own launch and ramp-up)
+    g = ctx.capture(stream, run_mfma)
+    replays = []
+    for _ in range(25):
+        e0, e1 = T.Event(), T.Event()
+        e0.record(stream)
+        g.launch(stream)
+        e1.record(stream)
+        e1.synchronize()
+        replays.append(e0.elapsed_ms(e1) * 1e3)
+        e0.destroy()
+        e1.destroy()
+    g.destroy()
+    graph_us = sorted(replays)[len(replays) // 2]
+    print(f"[bench] mfma graph replay median {graph_us:.1f} us", file=sys.stderr, flush=True)
     valu_steps = max(5, args.steps // 10)
     ms_valu = time_launches(run_valu, stream, valu_steps, max(2, args.warmup // 10))
     print(f"[bench] valu {ms_valu / valu_steps * 1e3:.1f} us/launch", file=sys.stderr, flush=True)
@@ -477,6 +493,7 @@ def main():
         "config": {"workload": "Saber L=3 MatrixVectorMul A.s + round q->p", "batch": args.batch,
                    "param_set": "Saber (q=2^13, p=2^10)", "backend": "mfma"},
         "valu": {"value": valu_value, "unit": "MatVec/s", "ms_per_step": ms_valu / valu_steps},
+        "mfma_graph_replay_median_us": graph_us,
         "mfma_over_valu": value / valu_value,
         "polymul_per_s": value * 9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
