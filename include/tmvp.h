@@ -12,9 +12,9 @@
  *  - Device pointers are plain HIP device addresses, owned by the caller.
  *    Work is enqueued on `stream` (NULL = the legacy default stream) and the
  *    call returns without waiting, as the reference's kernel launches do.
- *    The kernels read polynomials with 16-byte and byte strings with 4- and
- *    8-byte accesses: hipMalloc'd bases (256-byte aligned) run at full speed;
- *    other offsets are exact but slower.
+ *    The kernels access polynomials 16 bytes and byte strings 4 or 8 bytes
+ *    at a time, so device buffers must be 16-byte aligned (hipMalloc's bases
+ *    are 256-byte aligned).
  *  - Layouts are the reference's: polynomials are 256 consecutive uint16
  *    coefficients; A is [batch][L][L][256] row-major in (i, j); vectors are
  *    [batch][L][256]; one result polynomial is [batch][256].
