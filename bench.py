@@ -285,7 +285,7 @@ def mixed_bench(ctx, stream, batch=4096, steps=50):
 def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, seed=9):
     sets = ("saber", "sable2")
     rng = np.random.default_rng(seed)
-    srv = T.Server(ctx, max_batch=16384, slots=4)
+    srv = T.Server(ctx, max_batch=16384)  # the server's default of three slots
     pools = {}
     try:
         t0 = time.perf_counter()

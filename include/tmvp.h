@@ -248,7 +248,7 @@ typedef struct tmvp_server tmvp_server;
 typedef struct {
     int64_t hbm_budget; /* bytes for in-flight batches; 0 = 4 GiB */
     int64_t max_batch;  /* instances per batch; 0 = 16384 */
-    int slots;          /* concurrent batches at most; 0 = 4 */
+    int slots;          /* concurrent batches at most; 0 = 3 */
     int backend;        /* tmvp_backend */
 } tmvp_srv_config;
 
