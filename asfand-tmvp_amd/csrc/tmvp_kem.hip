@@ -441,13 +441,13 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
 }
 
 // Decapsulation, before the decryption and beside it (on the auxiliary
-// stream): the matrix stream from the pk inside the sk (pairs [t0, B); t0 = B
-// when A is resident) and H(ct) (pairs [B, 2B)).
+// stream): the matrix stream from the pk inside the sk (pairs [0, B)) and
+// H(ct) (pairs [B, 2B)).
 template <class SH>
 __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t *ct, const uint8_t *sk, uint16_t *A,
-                                                     uint8_t *hct, int64_t B, int64_t t0)
+                                                     uint8_t *hct, int64_t B)
 {
-    TMVP_PAIR(t0);
+    TMVP_PAIR(0);
     TMVP_RING;
     if (t >= 2 * B) return;
     if (t < B) {
@@ -464,15 +464,25 @@ __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t 
 
 // Decapsulation, after the decryption: m' from its bits, kr = G(m' | H(pk))
 // with H(pk) from the sk, secret from kr[32:64], b of the pk inside the sk to
-// V for the re-encryption, and the instance's mismatch flag cleared.
+// V for the re-encryption, and the instance's mismatch flag cleared (pairs
+// [0, B)).  With hct set, pairs [B, 2B) compute H(ct) beside it (the
+// decapsulation of keys whose A is resident has no matrix sponge to run it
+// next to).
 template <class SH>
 __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
                                                        uint8_t *buf, uint8_t *kr, uint16_t *S, uint16_t *V,
-                                                       uint32_t *diff, int64_t B)
+                                                       uint32_t *diff, int64_t B, const uint8_t *ct, uint8_t *hct)
 {
     TMVP_PAIR(0);
     TMVP_RING;
-    if (t >= B) return;
+    if (t >= (hct ? 2 * B : B)) return;
+    if (t >= B) {
+        const int64_t i = t - B;
+        uint32_t h[4];
+        sha3_256<1>(h, ct + i * P.ct, P.ct, p);
+        store_words(hct + i * 32, h, p);
+        return;
+    }
     if (p == 0) diff[t] = 0; // set by the re-encryption's compare-packs below
     uint32_t m[4];
     // this lane's 32 bits of each message word: bits 64w + 32p + j
@@ -900,15 +910,19 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
                                bool resident)
 {
     const int L = P.l;
-    const int64_t t0 = resident ? B : 0;
-    TRY(hipEventRecord(W.fork, st));
-    TRY(hipStreamWaitEvent(W.aux, W.fork, 0));
-    TRY(by_shape(P, [&](auto sh) {
-        hipLaunchKernelGGL(dec_pre_kernel<decltype(sh)>, dim3(pair_blocks(2 * B - t0)), dim3(64), 0, W.aux, P, ct, sk,
-                           W.A, W.hct, B, t0);
-        return hipGetLastError();
-    }));
-    TRY(hipEventRecord(W.join, W.aux));
+    // Without resident A, the matrix sponge and H(ct) (dec_pre_kernel) run on
+    // the side stream beside the decryption; with it, H(ct) runs inside the
+    // re-encryption's front kernel and the call stays on one stream.
+    if (!resident) {
+        TRY(hipEventRecord(W.fork, st));
+        TRY(hipStreamWaitEvent(W.aux, W.fork, 0));
+        TRY(by_shape(P, [&](auto sh) {
+            hipLaunchKernelGGL(dec_pre_kernel<decltype(sh)>, dim3(pair_blocks(2 * B)), dim3(64), 0, W.aux, P, ct, sk,
+                               W.A, W.hct, B);
+            return hipGetLastError();
+        }));
+        TRY(hipEventRecord(W.join, W.aux));
+    }
     // IND-CPA decryption (S/src/cuda_kernel.cu:341-370; SB/src/SABLE_indcpa.cu:422-463)
     if (P.sable)
         TRY(unpack(sk, P.sk, 0, W.s, L * kN, 2, L * kN, kXfSk2, B, st));
@@ -920,14 +934,14 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     TRY(product(P, backend, kInner, kEpiDec, W.bits, W.v2, W.s, W.poly, B, status, st));
     // re-encryption of m' under the pk stored in the sk
     TRY(by_shape(P, [&](auto sh) {
-        hipLaunchKernelGGL(dec_front_kernel<decltype(sh)>, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.bits, sk, W.buf,
-                           W.kr, W.s, W.v2, W.diff, B);
+        hipLaunchKernelGGL(dec_front_kernel<decltype(sh)>, dim3(pair_blocks(resident ? 2 * B : B)), dim3(64), 0, st, P,
+                           W.bits, sk, W.buf, W.kr, W.s, W.v2, W.diff, B, ct, resident ? W.hct : nullptr);
         return hipGetLastError();
     }));
     // ct is only read: the compare-packs never write through it
     uint8_t *ctc = const_cast<uint8_t *>(ct);
     TRY(enc_inner_half(P, W, backend, ctc, W.diff, B, status, st));
-    TRY(hipStreamWaitEvent(st, W.join, 0));
+    if (!resident) TRY(hipStreamWaitEvent(st, W.join, 0));
     TRY(enc_matvec_half(P, W, backend, ctc, W.diff, B, status, st));
     hipLaunchKernelGGL(dec_fin_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.diff, sk, W.hct, W.kr, ss, fail,
                        B);

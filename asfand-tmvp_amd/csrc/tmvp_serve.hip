@@ -118,8 +118,6 @@ __global__ __launch_bounds__(256) void copy_in_kernel(const CopyDesc *descs, int
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    hipStream_t aux = nullptr; // decapsulation side stream
-    hipEvent_t fork = nullptr, join = nullptr;
     uint8_t *ws = nullptr;    // KEM workspace
     uint8_t *d_in = nullptr;  // keys | inputs
     uint8_t *d_out = nullptr; // ct | ss | fail
@@ -184,11 +182,7 @@ void max_sizes(int64_t B, size_t &ws, size_t &in, size_t &out)
 void free_slot(Slot &s)
 {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
-    if (s.aux) (void)hipStreamSynchronize(s.aux);
     if (s.done) (void)hipEventDestroy(s.done);
-    if (s.fork) (void)hipEventDestroy(s.fork);
-    if (s.join) (void)hipEventDestroy(s.join);
-    if (s.aux) (void)hipStreamDestroy(s.aux);
     if (s.stream) (void)hipStreamDestroy(s.stream);
     if (s.ws) (void)hipFree(s.ws);
     if (s.d_in) (void)hipFree(s.d_in);
@@ -205,10 +199,6 @@ int alloc_slot(tmvp_server *S, Slot &s)
 {
     if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return TMVP_EHIP;
     if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) return TMVP_EHIP;
-    if (hipStreamCreateWithFlags(&s.aux, hipStreamNonBlocking) != hipSuccess) return TMVP_EHIP;
-    if (hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess)
-        return TMVP_EHIP;
     if (hipMalloc(&s.ws, S->ws_bytes) != hipSuccess || hipMalloc(&s.d_in, S->in_bytes) != hipSuccess ||
         hipMalloc(&s.d_out, S->out_bytes) != hipSuccess || hipMalloc(&s.d_status, sizeof(int)) != hipSuccess)
         return TMVP_ENOMEM;
@@ -269,9 +259,10 @@ int launch_slot(tmvp_server *S, Slot &s)
     if (hipGetLastError() != hipSuccess) return TMVP_EHIP;
     KemWorkspace W;
     kem_workspace_carve(P, B, s.ws, W);
-    W.aux = s.aux;
-    W.fork = s.fork;
-    W.join = s.join;
+    // keyed calls run on the slot's one stream (A is resident, so the
+    // decapsulation has no side-stream work)
+    W.aux = nullptr;
+    W.fork = W.join = nullptr;
     const Store &st = S->store[s.ps];
     KeyStore K{st.pk, st.sk, st.A, st.hpk};
     const int32_t *dkeys = reinterpret_cast<const int32_t *>(s.d_in);
@@ -511,9 +502,8 @@ int tmvp_srv_create(tmvp_ctx *ctx, const tmvp_srv_config *cfg, tmvp_server **out
     if (c.hbm_budget == 0) c.hbm_budget = (int64_t)4 << 30;
     if (c.max_batch == 0) c.max_batch = 16384;
     // Three concurrent batches by default: at 6000 req/s three slots held
-    // p50 0.38-0.39 ms / p99 0.85-0.89 ms over repeated runs, while four
-    // slots gave 0.71 / 1.6 ms in two of three runs (each slot is two
-    // streams, and the runtime spreads streams over four hardware queues).
+    // p50 0.35 ms / p99 0.75-0.78 ms over repeated runs, four 0.37-0.41 /
+    // 0.95-0.99 ms and six 0.37 / 0.90-1.0 ms.
     if (c.slots == 0) c.slots = 3;
     if (c.max_batch > ((int64_t)1 << 24)) return TMVP_EARG;
     int dev = 0;
