@@ -17,12 +17,13 @@
 //    arrival order up to max_batch instances, and retires finished slots by
 //    polling their events.  Requests arriving meanwhile join the next batch
 //    that a free slot takes.
-//  - Inputs reach the device through one copy-in kernel per batch that reads
-//    page-locked host memory directly: the key ids and pageable inputs from
-//    the slot's pinned staging, pinned inputs from the caller's own buffer.
-//    Submitting host-to-device DMA copies instead occasionally blocked the
-//    submitting thread for 7-11 ms inside the runtime, which stalled every
-//    slot.  Results return by DMA (device-to-host copies never blocked).
+//  - Data crosses the bus through a copy kernel that reads or writes
+//    page-locked host memory directly, once per batch each way: the key ids
+//    and pageable inputs from the slot's pinned staging, pinned inputs from
+//    the caller's own buffer; results into the caller's pinned buffers, else
+//    into the staging.  Submitting DMA copies instead occasionally blocked the
+//    submitting thread for 7-15 ms inside the runtime, which stalled every
+//    slot.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -62,6 +63,7 @@ struct Pending {
     double t_submit;
     bool pinned_in, pinned_out; // host buffers the device can reach directly
     const uint8_t *dev_in;      // pinned_in: the device's address of req.in
+    uint8_t *dev_ct, *dev_ss, *dev_fail; // pinned_out: the device's addresses of the outputs
 };
 
 // Whether p lies in page-locked host memory from hipHostMalloc; *dev gets
@@ -77,18 +79,20 @@ bool is_pinned(const void *p, const void **dev = nullptr)
     return ok;
 }
 
-// One piece of a batch's input: `bytes` from src (page-locked host memory,
-// read over the bus) to dst (device memory).
+// One piece of a batch's data: `bytes` from src to dst, one of them in
+// page-locked host memory (read or written over the bus).
 struct CopyDesc {
     const uint8_t *src;
     uint8_t *dst;
     uint32_t bytes;
 };
-constexpr uint32_t kPiece = 64u << 10; // bytes per descriptor: one workgroup each
+// Bytes per descriptor, one workgroup each: small pieces put many workgroups
+// on a request's bytes (64 KiB pieces raised p50 by about 30 us).
+constexpr uint32_t kPiece = 8u << 10;
 
 // Workgroup b copies piece b.  16-byte accesses when both ends and the size
 // allow, else 4-byte, else bytes; each thread keeps four loads in flight.
-__global__ __launch_bounds__(256) void copy_in_kernel(const CopyDesc *descs, int n)
+__global__ __launch_bounds__(256) void copy_kernel(const CopyDesc *descs, int n)
 {
     if ((int)blockIdx.x >= n) return;
     const CopyDesc d = descs[blockIdx.x];
@@ -123,8 +127,10 @@ struct Slot {
     uint8_t *d_out = nullptr; // ct | ss | fail
     int *d_status = nullptr;
     uint8_t *h_in = nullptr, *h_out = nullptr; // pinned staging
-    const uint8_t *h_in_dev = nullptr;         // the device's address of h_in
-    CopyDesc *h_desc = nullptr;                // pinned: the copy-in pieces
+    const uint8_t *h_in_dev = nullptr;         // the device's addresses of h_in,
+    uint8_t *h_out_dev = nullptr;              // h_out
+    int *h_status_dev = nullptr;               // and h_status
+    CopyDesc *h_desc = nullptr;                // pinned: the copy pieces of a batch
     const CopyDesc *h_desc_dev = nullptr;
     int *h_status = nullptr;
     bool busy = false;
@@ -210,6 +216,10 @@ int alloc_slot(tmvp_server *S, Slot &s)
     const void *dv = nullptr;
     if (!is_pinned(s.h_in, &dv)) return TMVP_EHIP;
     s.h_in_dev = static_cast<const uint8_t *>(dv);
+    if (!is_pinned(s.h_out, &dv)) return TMVP_EHIP;
+    s.h_out_dev = static_cast<uint8_t *>(const_cast<void *>(dv));
+    if (!is_pinned(s.h_status, &dv)) return TMVP_EHIP;
+    s.h_status_dev = static_cast<int *>(const_cast<void *>(dv));
     if (!is_pinned(s.h_desc, &dv)) return TMVP_EHIP;
     s.h_desc_dev = static_cast<const CopyDesc *>(dv);
     if (hipMemset(s.d_status, 0, sizeof(int)) != hipSuccess) return TMVP_EHIP;
@@ -255,7 +265,7 @@ int launch_slot(tmvp_server *S, Slot &s)
         off += p.req.count;
     }
     if (!fits) return TMVP_EARG; // cannot happen: desc_cap covers max_batch requests
-    hipLaunchKernelGGL(copy_in_kernel, dim3((unsigned)nd), dim3(256), 0, s.stream, s.h_desc_dev, nd);
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)nd), dim3(256), 0, s.stream, s.h_desc_dev, nd);
     if (hipGetLastError() != hipSuccess) return TMVP_EHIP;
     KemWorkspace W;
     kem_workspace_carve(P, B, s.ws, W);
@@ -278,7 +288,9 @@ int launch_slot(tmvp_server *S, Slot &s)
         e = kem_decaps_keyed(P, W, backend, K, dkeys, din, s.d_out, s.d_out + (size_t)B * 32, B, s.d_status,
                              s.stream);
     if (e != hipSuccess) return TMVP_EHIP;
-    // results: straight into pinned request buffers, else into the staging
+    // results (and the range flag): one copy kernel writes them straight into
+    // pinned request buffers, else into the staging
+    const int nd_in = nd;
     bool any_staged = false;
     const bool enc = s.op == TMVP_SRV_ENCAPS;
     const uint8_t *dss = enc ? s.d_out + (size_t)B * P.ct : s.d_out, *dfail = s.d_out + (size_t)B * 32;
@@ -288,21 +300,19 @@ int launch_slot(tmvp_server *S, Slot &s)
         if (!p.pinned_out) {
             any_staged = true;
         } else {
-            if (enc && hipMemcpyAsync(p.req.ct, s.d_out + off * P.ct, n * P.ct, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
-                return TMVP_EHIP;
-            if (hipMemcpyAsync(p.req.ss, dss + off * 32, n * 32, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
-                return TMVP_EHIP;
-            if (!enc && p.req.fail &&
-                hipMemcpyAsync(p.req.fail, dfail + off, n, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
-                return TMVP_EHIP;
+            if (enc) fits = fits && add(s.d_out + off * P.ct, p.dev_ct, n * P.ct);
+            fits = fits && add(dss + off * 32, p.dev_ss, n * 32);
+            if (!enc && p.req.fail) fits = fits && add(dfail + off, p.dev_fail, n);
         }
         off += (int64_t)n;
     }
-    if (any_staged && hipMemcpyAsync(s.h_out, s.d_out, out_total, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
-        return TMVP_EHIP;
-    if (hipMemcpyAsync(s.h_status, s.d_status, sizeof(int), hipMemcpyDeviceToHost, s.stream) != hipSuccess ||
-        hipEventRecord(s.done, s.stream) != hipSuccess)
-        return TMVP_EHIP;
+    if (any_staged) fits = fits && add(s.d_out, s.h_out_dev, out_total);
+    fits = fits && add(reinterpret_cast<const uint8_t *>(s.d_status), reinterpret_cast<uint8_t *>(s.h_status_dev),
+                       sizeof(int));
+    if (!fits) return TMVP_EARG; // cannot happen: desc_cap covers max_batch requests
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)(nd - nd_in)), dim3(256), 0, s.stream, s.h_desc_dev + nd_in,
+                       nd - nd_in);
+    if (hipGetLastError() != hipSuccess || hipEventRecord(s.done, s.stream) != hipSuccess) return TMVP_EHIP;
     return TMVP_OK;
 }
 
@@ -516,8 +526,10 @@ int tmvp_srv_create(tmvp_ctx *ctx, const tmvp_srv_config *cfg, tmvp_server **out
     S->max_batch = c.max_batch;
     S->t0 = Clock::now();
     max_sizes(c.max_batch, S->ws_bytes, S->in_bytes, S->out_bytes);
-    // every request has at least one instance: pieces <= requests + bytes / kPiece, plus the keys
-    S->desc_cap = 2 * c.max_batch + (int64_t)(S->in_bytes / kPiece) + 4;
+    // every request has at least one instance: inputs take at most one piece
+    // per request plus in_bytes / kPiece and the keys' pieces, outputs three
+    // per request plus out_bytes / kPiece, the staging and the flag
+    S->desc_cap = 5 * c.max_batch + (int64_t)((S->in_bytes + S->out_bytes) / kPiece) + 8;
     const int64_t per_slot = (int64_t)(S->ws_bytes + S->in_bytes + S->out_bytes);
     int nslots = (int)(c.hbm_budget / per_slot);
     if (nslots > c.slots) nslots = c.slots;
@@ -581,10 +593,10 @@ int tmvp_srv_submit(tmvp_server *S, const tmvp_srv_request *r, uint64_t *id)
     if (r->param_set < 0 || r->param_set >= TMVP_NUM_PARAM_SETS) return TMVP_EARG;
     if (r->count < 1 || !r->keys || !r->in || !r->ss || (r->op == TMVP_SRV_ENCAPS && !r->ct)) return TMVP_EARG;
     // outside the lock: pointer queries of the caller's buffers
-    const void *dev_in = nullptr;
+    const void *dev_in = nullptr, *dev_ct = nullptr, *dev_ss = nullptr, *dev_fail = nullptr;
     const bool pinned_in = is_pinned(r->in, &dev_in);
-    const bool pinned_out = is_pinned(r->ss) && (r->op != TMVP_SRV_ENCAPS || is_pinned(r->ct)) &&
-                            (r->op != TMVP_SRV_DECAPS || is_pinned(r->fail));
+    const bool pinned_out = is_pinned(r->ss, &dev_ss) && (r->op != TMVP_SRV_ENCAPS || is_pinned(r->ct, &dev_ct)) &&
+                            (r->op != TMVP_SRV_DECAPS || is_pinned(r->fail, &dev_fail));
     std::lock_guard<std::mutex> g(S->mu);
     if (r->count > S->max_batch) return TMVP_EARG;
     const Store &st = S->store[r->param_set];
@@ -598,6 +610,9 @@ int tmvp_srv_submit(tmvp_server *S, const tmvp_srv_request *r, uint64_t *id)
     p.pinned_in = pinned_in;
     p.pinned_out = pinned_out;
     p.dev_in = static_cast<const uint8_t *>(dev_in);
+    p.dev_ct = static_cast<uint8_t *>(const_cast<void *>(dev_ct));
+    p.dev_ss = static_cast<uint8_t *>(const_cast<void *>(dev_ss));
+    p.dev_fail = static_cast<uint8_t *>(const_cast<void *>(dev_fail));
     S->queue[r->param_set][r->op].push_back(p);
     S->pending_requests += 1;
     S->stats.requests_submitted += 1;

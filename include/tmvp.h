@@ -237,10 +237,9 @@ int tmvp_kem_reserve(tmvp_ctx *ctx, int param_set, int64_t batch, tmvp_stream_t 
  * a free slot (a stream with its own workspace and staging).  The HBM budget
  * sets the number of slots, so admission waits for budget; batches on
  * different slots overlap on the device.  Host buffers of a request must stay
- * valid until its completion is polled; inputs in page-locked memory from
- * hipHostMalloc are read by the device in place and outputs there written by
- * DMA directly; other buffers go through the server's pinned staging,
- * copied by the scheduler thread.  Results are those of
+ * valid until its completion is polled; buffers in page-locked memory from
+ * hipHostMalloc are read and written by the device in place; other buffers
+ * go through the server's pinned staging, copied by the scheduler thread.  Results are those of
  * tmvp_kem_encaps / tmvp_kem_decaps for the same keys and inputs, whatever
  * the batch a request lands in.  Run the serving process with
  * GPU_MAX_HW_QUEUES=8 (HIP's default is 4) so that every slot's stream has
