@@ -203,11 +203,15 @@ def test_serving_pinned_between_pageable(ctx, keys):
                 inputs.append(pdata)
             else:
                 inputs.append(data.copy())
+        # a large request first keeps the one slot busy while the others queue,
+        # so each (set, operation) forms one batch: pageable, pinned, pageable
+        blocker_ids = rng.integers(0, NKEYS, 4000).astype(np.int32)
+        srv.submit(T.SRV_ENCAPS, "saber", blocker_ids, rng.integers(0, 256, (4000, 32), dtype=np.uint8), tag=99)
         outs = {}
         for k, (op, ps, ids, data, want) in enumerate(reqs):
             _, outs[k] = srv.submit(op, ps, ids, inputs[k], tag=k)
         done = {}
-        while len(done) < len(reqs):
+        while len(done) < len(reqs) + 1:
             for c in srv.poll(64, timeout_us=200000):
                 done[c["tag"]] = c
         assert all(c["status"] == 0 for c in done.values())
