@@ -510,18 +510,17 @@ __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16
 // 64-bit words of a W-bit little-endian bit stream (LSB first, as every
 // reference codec: S/src/pack.cu:1-213, SB/src/pack.cu:182-467).  W is a
 // template argument so every shift and word index is a constant.
-// kXfSk2 is the Sable secret-key code: s & 3 on packing, ((x ^ 2) - 2) on
-// unpacking (SB/src/pack.cu:366-377).
+// kXfSk2 is the Sable secret-key code on unpacking, ((x ^ 2) - 2)
+// (SB/src/pack.cu:366-377).
 enum Xf : int { kXfNone = 0, kXfSk2 = 1 };
 
-// With dst2 set, the packed words also go to dst2 + t * dst2_stride (+ the
-// same chunk offset).  With diff set, nothing is written: the packed words are
-// compared with those already at dst, and a mismatch sets diff[t] to 1
-// (decapsulation's check of the re-encryption against the ciphertext).
-template <int W, int XF>
+// With diff set, nothing is written: the packed words are compared with those
+// already at dst, and a mismatch sets diff[t] to 1 (decapsulation's check of
+// the re-encryption against the ciphertext).
+template <int W>
 __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t src_stride, uint8_t *dst,
                                                    int64_t dst_stride, int dst_off, int chunks, int64_t B,
-                                                   uint8_t *dst2, int64_t dst2_stride, uint32_t *diff)
+                                                   uint32_t *diff)
 {
     const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (gid >= B * chunks) return;
@@ -540,7 +539,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t 
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int bit = (8 * q + j) * W, wi = bit >> 6, o = bit & 63;
-            const uint64_t x = (XF == kXfSk2 ? (e[j] & 3u) : e[j]) & mask;
+            const uint64_t x = e[j] & mask;
             words[wi] |= x << o;
             if (o + W > 64) words[wi + 1] |= x >> (64 - o);
         }
@@ -555,11 +554,6 @@ __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t 
     }
 #pragma unroll
     for (int k = 0; k < W; k++) d[k] = words[k];
-    if (dst2) {
-        uint64_t *d2 = reinterpret_cast<uint64_t *>(dst2 + t * dst2_stride + dst_off + (int64_t)c * W * 8);
-#pragma unroll
-        for (int k = 0; k < W; k++) d2[k] = words[k];
-    }
 }
 
 template <int W, int XF>
@@ -738,22 +732,14 @@ static hipError_t product(const KemParams &P, int backend, int mode, int epi, ui
 // Codec launch for a run-time width: finds the instantiation with W == w.
 template <int W>
 static hipError_t pack_w(const uint16_t *src, int64_t src_stride, uint8_t *dst, int64_t dst_stride, int dst_off, int w,
-                         int chunks, int xf, int64_t B, hipStream_t st, uint8_t *dst2, int64_t dst2_stride,
-                         uint32_t *diff)
+                         int chunks, int64_t B, hipStream_t st, uint32_t *diff)
 {
     if constexpr (W > 13) {
         return hipErrorInvalidValue;
     } else {
-        if (w != W)
-            return pack_w<W + 1>(src, src_stride, dst, dst_stride, dst_off, w, chunks, xf, B, st, dst2, dst2_stride,
-                                 diff);
-        const dim3 grid(blocks_for(B * chunks, 256));
-        if (xf == kXfSk2)
-            hipLaunchKernelGGL((pack_kernel<W, kXfSk2>), grid, dim3(256), 0, st, src, src_stride, dst, dst_stride,
-                               dst_off, chunks, B, dst2, dst2_stride, diff);
-        else
-            hipLaunchKernelGGL((pack_kernel<W, kXfNone>), grid, dim3(256), 0, st, src, src_stride, dst, dst_stride,
-                               dst_off, chunks, B, dst2, dst2_stride, diff);
+        if (w != W) return pack_w<W + 1>(src, src_stride, dst, dst_stride, dst_off, w, chunks, B, st, diff);
+        hipLaunchKernelGGL((pack_kernel<W>), dim3(blocks_for(B * chunks, 256)), dim3(256), 0, st, src, src_stride,
+                           dst, dst_stride, dst_off, chunks, B, diff);
         return hipGetLastError();
     }
 }
@@ -779,10 +765,9 @@ static hipError_t unpack_w(const uint8_t *src, int64_t src_stride, int src_off, 
 
 // ncoef is a multiple of 64; byte offsets and strides are multiples of 8.
 static hipError_t pack(const uint16_t *src, int64_t src_stride, uint8_t *dst, int64_t dst_stride, int dst_off, int w,
-                       int ncoef, int xf, int64_t B, hipStream_t st, uint8_t *dst2 = nullptr,
-                       int64_t dst2_stride = 0, uint32_t *diff = nullptr)
+                       int ncoef, int64_t B, hipStream_t st, uint32_t *diff = nullptr)
 {
-    return pack_w<1>(src, src_stride, dst, dst_stride, dst_off, w, ncoef / 64, xf, B, st, dst2, dst2_stride, diff);
+    return pack_w<1>(src, src_stride, dst, dst_stride, dst_off, w, ncoef / 64, B, st, diff);
 }
 
 static hipError_t unpack(const uint8_t *src, int64_t src_stride, int src_off, uint16_t *dst, int64_t dst_stride,
@@ -819,7 +804,7 @@ static hipError_t enc_inner_half(const KemParams &P, const KemWorkspace &W, int 
                                  uint32_t *diff, int64_t B, int *status, hipStream_t st)
 {
     TRY(product(P, backend, kInner, kEpiEnc, W.poly, W.v2, W.s, W.bits, B, status, st));
-    TRY(pack(W.poly, kN, ct, P.ct, P.pvc, P.et, kN, kXfNone, B, st, nullptr, 0, diff));
+    TRY(pack(W.poly, kN, ct, P.ct, P.pvc, P.et, kN, B, st, diff));
     return hipSuccess;
 }
 
@@ -828,7 +813,7 @@ static hipError_t enc_matvec_half(const KemParams &P, const KemWorkspace &W, int
 {
     const int L = P.l;
     TRY(product(P, backend, kMatVec, kEpiQ2P, W.v1, W.A, W.s, nullptr, B, status, st));
-    TRY(pack(W.v1, L * kN, ct, P.ct, 0, P.ep, L * kN, kXfNone, B, st, nullptr, 0, diff));
+    TRY(pack(W.v1, L * kN, ct, P.ct, 0, P.ep, L * kN, B, st, diff));
     return hipSuccess;
 }
 
