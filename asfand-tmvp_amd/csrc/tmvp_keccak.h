@@ -292,12 +292,5 @@ template <int U = 1, class F> __device__ __forceinline__ void squeeze_each(uint3
     }
 }
 
-// Squeeze `nwords` words to dst, each lane writing its halves.
-template <int U = 1> __device__ __forceinline__ void squeeze_words(uint32_t (&A)[25], uint8_t *dst, int nwords, int p)
-{
-    uint32_t *d = reinterpret_cast<uint32_t *>(dst) + p;
-    squeeze_each<U>(A, nwords, p, [&](int k, uint32_t h) { d[2 * k] = h; });
-}
-
 } // namespace keccak
 } // namespace tmvp
