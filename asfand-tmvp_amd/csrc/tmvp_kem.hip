@@ -586,16 +586,41 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *src, int64_t
     }
 }
 
-// dst[t] = src[keys[t]] for `words` 8-byte words per row (key ids checked by the caller).
-__global__ __launch_bounds__(256) void gather_kernel(const uint8_t *src, int64_t src_stride, const int32_t *keys,
-                                                     uint8_t *dst, int64_t dst_stride, int words, int64_t B)
+// Up to three gathers of one batch in one launch: segment g copies words[g]
+// 8-byte words per instance, dst[g] + t * dst_stride[g] <- src[g] +
+// keys[t] * src_stride[g] (key ids checked by the caller).  Unused segments
+// have zero words.
+struct GatherSet {
+    const uint8_t *src[3];
+    int64_t src_stride[3];
+    uint8_t *dst[3];
+    int64_t dst_stride[3];
+    int words[3];
+};
+
+__global__ __launch_bounds__(256) void gather_kernel(GatherSet G, const int32_t *keys, int64_t B)
 {
+    const int total = G.words[0] + G.words[1] + G.words[2];
     const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= B * words) return;
-    const int64_t t = gid / words;
-    const int k = (int)(gid - t * words);
-    reinterpret_cast<uint64_t *>(dst + t * dst_stride)[k] =
-        reinterpret_cast<const uint64_t *>(src + (int64_t)keys[t] * src_stride)[k];
+    if (gid >= B * total) return;
+    const int64_t t = gid / total;
+    int k = (int)(gid - t * total);
+    const uint8_t *src;
+    uint8_t *dst;
+    const int64_t key = keys[t];
+    if (k < G.words[0]) {
+        src = G.src[0] + key * G.src_stride[0];
+        dst = G.dst[0] + t * G.dst_stride[0];
+    } else if (k < G.words[0] + G.words[1]) {
+        k -= G.words[0];
+        src = G.src[1] + key * G.src_stride[1];
+        dst = G.dst[1] + t * G.dst_stride[1];
+    } else {
+        k -= G.words[0] + G.words[1];
+        src = G.src[2] + key * G.src_stride[2];
+        dst = G.dst[2] + t * G.dst_stride[2];
+    }
+    reinterpret_cast<uint64_t *>(dst)[k] = reinterpret_cast<const uint64_t *>(src)[k];
 }
 
 // ---------------------------------------------------------------- hashing steps (one lane pair per instance)
@@ -940,25 +965,27 @@ hipError_t kem_decaps(const KemParams &P, const KemWorkspace &W, int backend, co
     return decaps_chain(P, W, backend, ct, sk, ss, fail, B, status, st, false);
 }
 
-static hipError_t gather(const uint8_t *src, int64_t src_stride, const int32_t *keys, uint8_t *dst, int64_t dst_stride,
-                         int bytes, int64_t B, hipStream_t st)
+// One launch for the gathers in G (bytes per row given in words[] as bytes,
+// multiples of 8).
+static hipError_t gather(GatherSet G, const int32_t *keys, int64_t B, hipStream_t st)
 {
-    const int words = bytes / 8;
-    hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(B * words, 256)), dim3(256), 0, st, src, src_stride, keys, dst,
-                       dst_stride, words, B);
+    for (int g = 0; g < 3; g++) G.words[g] /= 8;
+    const int total = G.words[0] + G.words[1] + G.words[2];
+    hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(B * total, 256)), dim3(256), 0, st, G, keys, B);
     return hipGetLastError();
 }
-
 hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backend, const KeyStore &K,
                             const int32_t *keys, const uint8_t *rnd, uint8_t *ct, uint8_t *ss, int64_t B, int *status,
                             hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
     const int abytes = P.l * P.l * kN * 2;
-    TRY(gather(K.pk, P.pk, keys, W.pkg, P.pk, P.pk, B, st));
-    TRY(gather(reinterpret_cast<const uint8_t *>(K.A), abytes, keys, reinterpret_cast<uint8_t *>(W.A), abytes, abytes,
-               B, st));
-    TRY(gather(K.hpk, 32, keys, W.hpkg, 32, 32, B, st));
+    TRY(gather(GatherSet{{K.pk, reinterpret_cast<const uint8_t *>(K.A), K.hpk},
+                         {P.pk, abytes, 32},
+                         {W.pkg, reinterpret_cast<uint8_t *>(W.A), W.hpkg},
+                         {P.pk, abytes, 32},
+                         {P.pk, abytes, 32}},
+               keys, B, st));
     TRY(by_shape(P, [&](auto sh) {
         hipLaunchKernelGGL(enc_front_kernel<decltype(sh)>, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg,
                            W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, B);
@@ -973,9 +1000,12 @@ hipError_t kem_decaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
 {
     if (B <= 0) return hipSuccess;
     const int abytes = P.l * P.l * kN * 2;
-    TRY(gather(K.sk, P.sk, keys, W.skg, P.sk, P.sk, B, st));
-    TRY(gather(reinterpret_cast<const uint8_t *>(K.A), abytes, keys, reinterpret_cast<uint8_t *>(W.A), abytes, abytes,
-               B, st));
+    TRY(gather(GatherSet{{K.sk, reinterpret_cast<const uint8_t *>(K.A), nullptr},
+                         {P.sk, abytes, 0},
+                         {W.skg, reinterpret_cast<uint8_t *>(W.A), nullptr},
+                         {P.sk, abytes, 0},
+                         {P.sk, abytes, 0}},
+               keys, B, st));
     return decaps_chain(P, W, backend, ct, W.skg, ss, fail, B, status, st, true);
 }
 
