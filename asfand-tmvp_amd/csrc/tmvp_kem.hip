@@ -556,20 +556,13 @@ __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t 
     for (int k = 0; k < W; k++) d[k] = words[k];
 }
 
-template <int W, int XF>
-__global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *src, int64_t src_stride, int src_off,
-                                                     uint16_t *dst, int64_t dst_stride, int chunks, int64_t B)
+// One chunk: W 64-bit words at s -> 64 coefficients at d.
+template <int W, int XF> __device__ __forceinline__ void unpack_chunk(const uint64_t *s, uint4 *d)
 {
-    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= B * chunks) return;
-    const int64_t t = gid / chunks;
-    const int c = (int)(gid - t * chunks);
-    const uint64_t *s = reinterpret_cast<const uint64_t *>(src + t * src_stride + src_off + (int64_t)c * W * 8);
     constexpr uint32_t mask = (1u << W) - 1u;
     uint64_t words[W];
 #pragma unroll
     for (int k = 0; k < W; k++) words[k] = s[k];
-    uint4 *d = reinterpret_cast<uint4 *>(dst + t * dst_stride + c * 64);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         uint32_t e[8];
@@ -584,6 +577,46 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *src, int64_t
         }
         d[q] = make_uint4(e[0] | (e[1] << 16), e[2] | (e[3] << 16), e[4] | (e[5] << 16), e[6] | (e[7] << 16));
     }
+}
+
+template <int W, int XF>
+__global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *src, int64_t src_stride, int src_off,
+                                                     uint16_t *dst, int64_t dst_stride, int chunks, int64_t B)
+{
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= B * chunks) return;
+    const int64_t t = gid / chunks;
+    const int c = (int)(gid - t * chunks);
+    unpack_chunk<W, XF>(reinterpret_cast<const uint64_t *>(src + t * src_stride + src_off + (int64_t)c * W * 8),
+                        reinterpret_cast<uint4 *>(dst + t * dst_stride + c * 64));
+}
+
+// Decapsulation's three decodes in one launch (S/src/cuda_kernel.cu:341-352;
+// SB/src/SABLE_indcpa.cu:422-440): per instance 4L chunks of s from the sk (WS
+// bits, or the Sable 2-bit code), 4L chunks of b' (WB bits) and 4 chunks of
+// c_m (WC bits) from the ct.  Used where nothing runs beside the decryption
+// (A resident); beside the matrix sponge, three launches measured faster.
+template <int WS, int XS, int WB, int WC>
+__global__ __launch_bounds__(256) void dec_unpack_kernel(const uint8_t *sk, int64_t sk_stride, const uint8_t *ct,
+                                                         int64_t ct_stride, int pvc, uint16_t *S, uint16_t *Bp,
+                                                         uint16_t *Cm, int l, int64_t B)
+{
+    const int cv = 4 * l; // chunks of one polynomial vector
+    const int chunks = 2 * cv + 4;
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= B * chunks) return;
+    const int64_t t = gid / chunks;
+    const int c = (int)(gid - t * chunks);
+    if (c < cv)
+        unpack_chunk<WS, XS>(reinterpret_cast<const uint64_t *>(sk + t * sk_stride + (int64_t)c * WS * 8),
+                             reinterpret_cast<uint4 *>(S + (t * cv + c) * 64));
+    else if (c < 2 * cv)
+        unpack_chunk<WB, kXfNone>(reinterpret_cast<const uint64_t *>(ct + t * ct_stride + (int64_t)(c - cv) * WB * 8),
+                                  reinterpret_cast<uint4 *>(Bp + (t * cv + c - cv) * 64));
+    else
+        unpack_chunk<WC, kXfNone>(
+            reinterpret_cast<const uint64_t *>(ct + t * ct_stride + pvc + (int64_t)(c - 2 * cv) * WC * 8),
+            reinterpret_cast<uint4 *>(Cm + t * kN + (c - 2 * cv) * 64));
 }
 
 // Up to three gathers of one batch in one launch: segment g copies words[g]
@@ -801,6 +834,33 @@ static hipError_t unpack(const uint8_t *src, int64_t src_stride, int src_off, ui
     return unpack_w<1>(src, src_stride, src_off, dst, dst_stride, w, ncoef / 64, xf, B, st);
 }
 
+template <int WS, int XS, int WB, int WC>
+static hipError_t dec_unpack_w(const KemParams &P, const uint8_t *sk, const uint8_t *ct, uint16_t *S, uint16_t *Bp,
+                               uint16_t *Cm, int64_t B, hipStream_t st)
+{
+    hipLaunchKernelGGL((dec_unpack_kernel<WS, XS, WB, WC>), dim3(blocks_for(B * (8 * P.l + 4), 256)), dim3(256), 0,
+                       st, sk, (int64_t)P.sk, ct, (int64_t)P.ct, P.pvc, S, Bp, Cm, P.l, B);
+    return hipGetLastError();
+}
+
+// s (from the sk), b' and c_m (from the ct) of a decapsulation in one launch,
+// for the six parameter sets' widths.
+static hipError_t dec_unpack(const KemParams &P, const uint8_t *sk, const uint8_t *ct, uint16_t *S, uint16_t *Bp,
+                             uint16_t *Cm, int64_t B, hipStream_t st)
+{
+    if (!P.sable && P.eq == 13 && P.ep == 10) {
+        if (P.et == 3) return dec_unpack_w<13, kXfNone, 10, 3>(P, sk, ct, S, Bp, Cm, B, st);
+        if (P.et == 4) return dec_unpack_w<13, kXfNone, 10, 4>(P, sk, ct, S, Bp, Cm, B, st);
+        if (P.et == 6) return dec_unpack_w<13, kXfNone, 10, 6>(P, sk, ct, S, Bp, Cm, B, st);
+    }
+    if (P.sable) {
+        if (P.ep == 9 && P.et == 3) return dec_unpack_w<2, kXfSk2, 9, 3>(P, sk, ct, S, Bp, Cm, B, st);
+        if (P.ep == 9 && P.et == 5) return dec_unpack_w<2, kXfSk2, 9, 5>(P, sk, ct, S, Bp, Cm, B, st);
+        if (P.ep == 10 && P.et == 3) return dec_unpack_w<2, kXfSk2, 10, 3>(P, sk, ct, S, Bp, Cm, B, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 // Calls f(Shape<EQ, MU, EP>{}) for the parameter set's sampling shape.
 template <class F> static hipError_t by_shape(const KemParams &P, F f)
 {
@@ -934,12 +994,16 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
         TRY(hipEventRecord(W.join, W.aux));
     }
     // IND-CPA decryption (S/src/cuda_kernel.cu:341-370; SB/src/SABLE_indcpa.cu:422-463)
-    if (P.sable)
-        TRY(unpack(sk, P.sk, 0, W.s, L * kN, 2, L * kN, kXfSk2, B, st));
-    else
-        TRY(unpack(sk, P.sk, 0, W.s, L * kN, P.eq, L * kN, kXfNone, B, st));
-    TRY(unpack(ct, P.ct, 0, W.v2, L * kN, P.ep, L * kN, kXfNone, B, st));
-    TRY(unpack(ct, P.ct, P.pvc, W.poly, kN, P.et, kN, kXfNone, B, st));
+    if (resident) {
+        TRY(dec_unpack(P, sk, ct, W.s, W.v2, W.poly, B, st));
+    } else {
+        if (P.sable)
+            TRY(unpack(sk, P.sk, 0, W.s, L * kN, 2, L * kN, kXfSk2, B, st));
+        else
+            TRY(unpack(sk, P.sk, 0, W.s, L * kN, P.eq, L * kN, kXfNone, B, st));
+        TRY(unpack(ct, P.ct, 0, W.v2, L * kN, P.ep, L * kN, kXfNone, B, st));
+        TRY(unpack(ct, P.ct, P.pvc, W.poly, kN, P.et, kN, kXfNone, B, st));
+    }
     // m' bits to W.bits, where the re-encryption's InnerProd reads them
     TRY(product(P, backend, kInner, kEpiDec, W.bits, W.v2, W.s, W.poly, B, status, st));
     // re-encryption of m' under the pk stored in the sk

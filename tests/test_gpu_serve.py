@@ -257,3 +257,37 @@ def test_serving_pinned_input_alignment(ctx, keys, offset):
     for k, (op, ps, ids, view, want) in enumerate(reqs):
         for got, exp in zip(outs[k], want):
             assert np.array_equal(got, exp), (k, op, ps, offset)
+
+
+def test_serving_all_parameter_sets(ctx):
+    """Keyed encapsulation and decapsulation through the server for every
+    parameter set give the plain batched calls' bytes."""
+    rng = np.random.default_rng(77)
+    sets = ("lightsaber", "saber", "firesaber", "sable1", "sable2", "sable3")
+    srv = T.Server(ctx, max_batch=1024, slots=2)
+    try:
+        want, outs = {}, {}
+        for i, ps in enumerate(sets):
+            pk, sk = T.run_kem_keypair(ctx, rng.integers(0, 256, (40, 96), dtype=np.uint8), ps, T.MFMA)
+            srv.add_keys(ps, pk, sk)
+            ids = rng.integers(0, 40, 37).astype(np.int32)
+            rnd = rng.integers(0, 256, (37, 32), dtype=np.uint8)
+            ct, ss = T.run_kem_encaps(ctx, pk[ids], rnd, ps, T.MFMA)
+            bad = ct.copy()
+            bad[::4, 5] ^= 0x40
+            want[(ps, 0)] = (ct, ss)
+            want[(ps, 1)] = T.run_kem_decaps(ctx, bad, sk[ids], ps, T.MFMA)
+            _, outs[(ps, 0)] = srv.submit(T.SRV_ENCAPS, ps, ids, rnd, tag=2 * i)
+            _, outs[(ps, 1)] = srv.submit(T.SRV_DECAPS, ps, ids, bad, tag=2 * i + 1)
+        done = {}
+        while len(done) < 2 * len(sets):
+            for c in srv.poll(64, timeout_us=200000):
+                done[c["tag"]] = c
+        assert all(c["status"] == 0 for c in done.values())
+    finally:
+        srv.close()
+    for key, exp in want.items():
+        for got, e in zip(outs[key], exp):
+            assert np.array_equal(got, e), key
+    for ps in sets:
+        assert outs[(ps, 1)][1][::4].all() and not outs[(ps, 1)][1][1::4].any(), ps
