@@ -1014,9 +1014,20 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     }));
     // ct is only read: the compare-packs never write through it
     uint8_t *ctc = const_cast<uint8_t *>(ct);
-    TRY(enc_inner_half(P, W, backend, ctc, W.diff, B, status, st));
-    if (!resident) TRY(hipStreamWaitEvent(st, W.join, 0));
-    TRY(enc_matvec_half(P, W, backend, ctc, W.diff, B, status, st));
+    if (resident && backend == 0) {
+        // A is there from the start: both products of the re-encryption as
+        // one L + 1 row product (as in encapsulation), then both compares
+        Job J = make_job(P, kMatVec, kEpiQ2P, W.v1, W.A, W.s, W.bits, B, status);
+        J.b = W.v2;
+        J.out2 = W.poly;
+        TRY(launch_mfma(J, kEnc, L, st));
+        TRY(pack(W.poly, kN, ctc, P.ct, P.pvc, P.et, kN, B, st, W.diff));
+        TRY(pack(W.v1, L * kN, ctc, P.ct, 0, P.ep, L * kN, B, st, W.diff));
+    } else {
+        TRY(enc_inner_half(P, W, backend, ctc, W.diff, B, status, st));
+        if (!resident) TRY(hipStreamWaitEvent(st, W.join, 0));
+        TRY(enc_matvec_half(P, W, backend, ctc, W.diff, B, status, st));
+    }
     hipLaunchKernelGGL(dec_fin_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.diff, sk, W.hct, W.kr, ss, fail,
                        B);
     return hipGetLastError();
