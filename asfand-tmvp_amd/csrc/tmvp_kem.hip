@@ -514,21 +514,10 @@ __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16
 // (SB/src/pack.cu:366-377).
 enum Xf : int { kXfNone = 0, kXfSk2 = 1 };
 
-// With diff set, nothing is written: the packed words are compared with those
-// already at dst, and a mismatch sets diff[t] to 1 (decapsulation's check of
-// the re-encryption against the ciphertext).
-template <int W>
-__global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t src_stride, uint8_t *dst,
-                                                   int64_t dst_stride, int dst_off, int chunks, int64_t B,
-                                                   uint32_t *diff)
+// One chunk: 64 coefficients at s -> W 64-bit words.
+template <int W> __device__ __forceinline__ void pack_chunk(const uint4 *s, uint64_t (&words)[W])
 {
-    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (gid >= B * chunks) return;
-    const int64_t t = gid / chunks;
-    const int c = (int)(gid - t * chunks);
-    const uint4 *s = reinterpret_cast<const uint4 *>(src + t * src_stride + c * 64);
     constexpr uint32_t mask = (1u << W) - 1u;
-    uint64_t words[W];
 #pragma unroll
     for (int k = 0; k < W; k++) words[k] = 0;
 #pragma unroll
@@ -544,6 +533,22 @@ __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t 
             if (o + W > 64) words[wi + 1] |= x >> (64 - o);
         }
     }
+}
+
+// With diff set, nothing is written: the packed words are compared with those
+// already at dst, and a mismatch sets diff[t] to 1 (decapsulation's check of
+// the re-encryption against the ciphertext).
+template <int W>
+__global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t src_stride, uint8_t *dst,
+                                                   int64_t dst_stride, int dst_off, int chunks, int64_t B,
+                                                   uint32_t *diff)
+{
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= B * chunks) return;
+    const int64_t t = gid / chunks;
+    const int c = (int)(gid - t * chunks);
+    uint64_t words[W];
+    pack_chunk<W>(reinterpret_cast<const uint4 *>(src + t * src_stride + c * 64), words);
     uint64_t *d = reinterpret_cast<uint64_t *>(dst + t * dst_stride + dst_off + (int64_t)c * W * 8);
     if (diff) {
         uint64_t x = 0;
@@ -554,6 +559,37 @@ __global__ __launch_bounds__(256) void pack_kernel(const uint16_t *src, int64_t 
     }
 #pragma unroll
     for (int k = 0; k < W; k++) d[k] = words[k];
+}
+
+// Decapsulation's check of the whole re-encryption in one launch: 4L chunks
+// of b' (bp, EP bits) against ct[0, pvc) and 4 chunks of v' (vp, ET bits)
+// against ct[pvc, ...); a mismatch sets diff[t] (S/src/cuda_kernel.cu:414,
+// :442, :445).
+template <int EP, int ET>
+__global__ __launch_bounds__(256) void dec_compare_kernel(const uint16_t *bp, const uint16_t *vp, const uint8_t *ct,
+                                                          int64_t ct_stride, int pvc, uint32_t *diff, int l, int64_t B)
+{
+    const int cv = 4 * l;
+    const int chunks = cv + 4;
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= B * chunks) return;
+    const int64_t t = gid / chunks;
+    const int c = (int)(gid - t * chunks);
+    uint64_t x = 0;
+    if (c < cv) {
+        uint64_t w[EP];
+        pack_chunk<EP>(reinterpret_cast<const uint4 *>(bp + (t * cv + c) * 64), w);
+        const uint64_t *d = reinterpret_cast<const uint64_t *>(ct + t * ct_stride + (int64_t)c * EP * 8);
+#pragma unroll
+        for (int k = 0; k < EP; k++) x |= d[k] ^ w[k];
+    } else {
+        uint64_t w[ET];
+        pack_chunk<ET>(reinterpret_cast<const uint4 *>(vp + t * kN + (c - cv) * 64), w);
+        const uint64_t *d = reinterpret_cast<const uint64_t *>(ct + t * ct_stride + pvc + (int64_t)(c - cv) * ET * 8);
+#pragma unroll
+        for (int k = 0; k < ET; k++) x |= d[k] ^ w[k];
+    }
+    if (x) atomicOr(diff + t, 1u);
 }
 
 // One chunk: W 64-bit words at s -> 64 coefficients at d.
@@ -843,6 +879,27 @@ static hipError_t dec_unpack_w(const KemParams &P, const uint8_t *sk, const uint
     return hipGetLastError();
 }
 
+template <int EP, int ET>
+static hipError_t dec_compare_w(const KemParams &P, const uint16_t *bp, const uint16_t *vp, const uint8_t *ct,
+                                uint32_t *diff, int64_t B, hipStream_t st)
+{
+    hipLaunchKernelGGL((dec_compare_kernel<EP, ET>), dim3(blocks_for(B * (4 * P.l + 4), 256)), dim3(256), 0, st, bp, vp,
+                       ct, (int64_t)P.ct, P.pvc, diff, P.l, B);
+    return hipGetLastError();
+}
+
+// The re-encryption (b' and v') against ct in one launch.
+static hipError_t dec_compare(const KemParams &P, const uint16_t *bp, const uint16_t *vp, const uint8_t *ct,
+                              uint32_t *diff, int64_t B, hipStream_t st)
+{
+    if (P.ep == 10 && P.et == 3) return dec_compare_w<10, 3>(P, bp, vp, ct, diff, B, st);
+    if (P.ep == 10 && P.et == 4) return dec_compare_w<10, 4>(P, bp, vp, ct, diff, B, st);
+    if (P.ep == 10 && P.et == 6) return dec_compare_w<10, 6>(P, bp, vp, ct, diff, B, st);
+    if (P.ep == 9 && P.et == 3) return dec_compare_w<9, 3>(P, bp, vp, ct, diff, B, st);
+    if (P.ep == 9 && P.et == 5) return dec_compare_w<9, 5>(P, bp, vp, ct, diff, B, st);
+    return hipErrorInvalidValue;
+}
+
 // s (from the sk), b' and c_m (from the ct) of a decapsulation in one launch,
 // for the six parameter sets' widths.
 static hipError_t dec_unpack(const KemParams &P, const uint8_t *sk, const uint8_t *ct, uint16_t *S, uint16_t *Bp,
@@ -1021,8 +1078,7 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
         J.b = W.v2;
         J.out2 = W.poly;
         TRY(launch_mfma(J, kEnc, L, st));
-        TRY(pack(W.poly, kN, ctc, P.ct, P.pvc, P.et, kN, B, st, W.diff));
-        TRY(pack(W.v1, L * kN, ctc, P.ct, 0, P.ep, L * kN, B, st, W.diff));
+        TRY(dec_compare(P, W.v1, W.poly, ct, W.diff, B, st));
     } else {
         TRY(enc_inner_half(P, W, backend, ctc, W.diff, B, status, st));
         if (!resident) TRY(hipStreamWaitEvent(st, W.join, 0));
