@@ -32,6 +32,7 @@ struct Job {
     const uint16_t *a;   // public operand (A, b or a)
     const uint16_t *b;   // kEnc: b of the public key [batch][L][256]
     uint16_t *out2;      // kEnc: the InnerProd row's output [batch][256]
+    const int32_t *akey; // kEnc: instance i reads the matrix a + akey[i] (null: a + i)
     const uint16_t *s;   // secret operand
     const uint16_t *aux; // message bits / scale polynomial (epilogues 2, 3)
     int *status;         // set to 1 when an MFMA launch meets an out-of-range secret

@@ -962,8 +962,11 @@ static hipError_t enc_matvec_half(const KemParams &P, const KemWorkspace &W, int
 // Encryption's products and the final hashes.  MFMA: A s' and b . s' as one
 // product, and the ciphertext packed inside the hash kernel; VALU: the two
 // halves, each packing its part, then the hash kernel.
+// A / akey: where the MFMA product reads each instance's matrix (default:
+// W.A, instance i at i; keyed calls: the key store at the key id).
 static hipError_t enc_finish(const KemParams &P, const KemWorkspace &W, int backend, uint8_t *ct, uint8_t *ss,
-                             int64_t B, int *status, hipStream_t st)
+                             int64_t B, int *status, hipStream_t st, const uint16_t *A = nullptr,
+                             const int32_t *akey = nullptr)
 {
     const dim3 grid(pair_blocks(B));
     if (backend != 0) {
@@ -972,9 +975,10 @@ static hipError_t enc_finish(const KemParams &P, const KemWorkspace &W, int back
         hipLaunchKernelGGL((enc_hash2_kernel<0, 0>), grid, dim3(64), 0, st, P, ct, W.kr, ss, B, nullptr, nullptr);
         return hipGetLastError();
     }
-    Job J = make_job(P, kMatVec, kEpiQ2P, W.v1, W.A, W.s, W.bits, B, status);
+    Job J = make_job(P, kMatVec, kEpiQ2P, W.v1, A ? A : W.A, W.s, W.bits, B, status);
     J.b = W.v2;
     J.out2 = W.poly;
+    J.akey = akey;
     TRY(launch_mfma(J, kEnc, P.l, st));
     auto go = [&](auto k) {
         constexpr int EP = decltype(k)::first_type::value, ET = decltype(k)::second_type::value;
@@ -1032,9 +1036,11 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
 // already in W.A and is not generated again.  The matrix stream (and its
 // unpacking) and H(ct) depend only on the inputs and run on W.aux beside the
 // decryption and the InnerProd half of the re-encryption.
+// resident: A is already in place (W.A, or the key store A at akey[i] for the
+// MFMA backend), so the matrix sponge is skipped.
 static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *ct,
                                const uint8_t *sk, uint8_t *ss, uint8_t *fail, int64_t B, int *status, hipStream_t st,
-                               bool resident)
+                               bool resident, const uint16_t *A = nullptr, const int32_t *akey = nullptr)
 {
     const int L = P.l;
     // Without resident A, the matrix sponge and H(ct) (dec_pre_kernel) run on
@@ -1074,9 +1080,10 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     if (resident && backend == 0) {
         // A is there from the start: both products of the re-encryption as
         // one L + 1 row product (as in encapsulation), then both compares
-        Job J = make_job(P, kMatVec, kEpiQ2P, W.v1, W.A, W.s, W.bits, B, status);
+        Job J = make_job(P, kMatVec, kEpiQ2P, W.v1, A ? A : W.A, W.s, W.bits, B, status);
         J.b = W.v2;
         J.out2 = W.poly;
+        J.akey = akey;
         TRY(launch_mfma(J, kEnc, L, st));
         TRY(dec_compare(P, W.v1, W.poly, ct, W.diff, B, st));
     } else {
@@ -1110,18 +1117,21 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
                             hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
-    const int abytes = P.l * P.l * kN * 2;
-    TRY(gather(GatherSet{{K.pk, reinterpret_cast<const uint8_t *>(K.A), K.hpk},
-                         {P.pk, abytes, 32},
-                         {W.pkg, reinterpret_cast<uint8_t *>(W.A), W.hpkg},
-                         {P.pk, abytes, 32},
-                         {P.pk, abytes, 32}},
+    // the MFMA product reads A from the key store at the key id; the VALU
+    // backend gets a gathered copy
+    const int abytes = backend == 0 ? 0 : P.l * P.l * kN * 2;
+    TRY(gather(GatherSet{{K.pk, K.hpk, reinterpret_cast<const uint8_t *>(K.A)},
+                         {P.pk, 32, abytes},
+                         {W.pkg, W.hpkg, reinterpret_cast<uint8_t *>(W.A)},
+                         {P.pk, 32, abytes},
+                         {P.pk, 32, abytes}},
                keys, B, st));
     TRY(by_shape(P, [&](auto sh) {
         hipLaunchKernelGGL(enc_front_kernel<decltype(sh)>, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg,
                            W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, B);
         return hipGetLastError();
     }));
+    if (backend == 0) return enc_finish(P, W, backend, ct, ss, B, status, st, K.A, keys);
     return enc_finish(P, W, backend, ct, ss, B, status, st);
 }
 
@@ -1130,13 +1140,15 @@ hipError_t kem_decaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
                             int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
-    const int abytes = P.l * P.l * kN * 2;
+    // as in kem_encaps_keyed: A from the key store for the MFMA product
+    const int abytes = backend == 0 ? 0 : P.l * P.l * kN * 2;
     TRY(gather(GatherSet{{K.sk, reinterpret_cast<const uint8_t *>(K.A), nullptr},
                          {P.sk, abytes, 0},
                          {W.skg, reinterpret_cast<uint8_t *>(W.A), nullptr},
                          {P.sk, abytes, 0},
                          {P.sk, abytes, 0}},
                keys, B, st));
+    if (backend == 0) return decaps_chain(P, W, backend, ct, W.skg, ss, fail, B, status, st, true, K.A, keys);
     return decaps_chain(P, W, backend, ct, W.skg, ss, fail, B, status, st, true);
 }
 

@@ -191,7 +191,10 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     constexpr int NPA = (MODE == kEnc) ? L * L : NPOLY;
     auto fetch_term = [&](int64_t i, int j) {
         const int64_t ie = i < J.batch ? i : inst;
-        const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + ie * (NPA * kN));
+        int64_t ia = ie; // the instance's matrix (kEnc may read it from a key store)
+        if constexpr (MODE == kEnc)
+            if (J.akey) ia = J.akey[ie];
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + ia * (NPA * kN));
 #pragma unroll
         for (int it = 0; it < ITJ; it++) {
             const int u0 = lane + 64 * it;                             // chunk of this term: row u >> 5, piece u & 31
