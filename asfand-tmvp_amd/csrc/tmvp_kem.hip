@@ -403,17 +403,19 @@ __device__ __forceinline__ void enc_matrix_role(const KemParams &P, const uint8_
 template <class SH, int U>
 __device__ __forceinline__ void enc_secret_role(const KemParams &P, const uint8_t *rnd, const uint8_t *pk,
                                                 const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *S,
-                                                uint16_t *V, uint16_t *bits, int64_t t, uint32_t *ring, int p)
+                                                uint16_t *V, uint16_t *bits, int64_t t, int64_t kt, uint32_t *ring,
+                                                int p)
 {
+    // t: the instance; kt: its row of pk / hpk (a key id when they are a key store)
     uint32_t r[4], m[4], h[4], coins[4];
     load_words(r, rnd + t * 32, p);
     sha3_256_words<4, U>(m, r, p);
-    if (hpk) load_words(h, hpk + t * 32, p);
-    else sha3_256<U>(h, pk + t * P.pk, P.pk, p);
+    if (hpk) load_words(h, hpk + kt * 32, p);
+    else sha3_256<U>(h, pk + kt * P.pk, P.pk, p);
     hash_g<U>(m, h, buf + t * 64, kr + t * 64, coins, p);
     secret_stream<SH::MU, U>(P, coins, S + t * P.l * kN, ring, p);
     message_bits(m, bits + t * kN, p);
-    unpack_b<SH::EP>(P, pk + t * P.pk, V + t * P.l * kN, p);
+    unpack_b<SH::EP>(P, pk + kt * P.pk, V + t * P.l * kN, p);
 }
 
 template <class SH>
@@ -431,13 +433,18 @@ template <class SH>
 __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
                                                        const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *A,
                                                        uint16_t *S, uint16_t *V, uint16_t *bits,
-                                                       int64_t B, int64_t t0)
+                                                       int64_t B, int64_t t0, const int32_t *keys)
 {
     TMVP_PAIR(t0);
     TMVP_RING;
     if (t >= 2 * B) return;
-    if (t < B) enc_matrix_role<SH, 2>(P, pk, A, t, ring, p);
-    else enc_secret_role<SH, 2>(P, rnd, pk, hpk, buf, kr, S, V, bits, t - B, ring, p);
+    if (t < B) {
+        enc_matrix_role<SH, 2>(P, pk, A, t, ring, p);
+    } else {
+        // keys (A resident, t0 = B): pk and hpk are key stores, read at the key id
+        const int64_t i = t - B;
+        enc_secret_role<SH, 2>(P, rnd, pk, hpk, buf, kr, S, V, bits, i, keys ? (int64_t)keys[i] : i, ring, p);
+    }
 }
 
 // Decapsulation, before the decryption and beside it (on the auxiliary
@@ -1026,7 +1033,7 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
     if (B <= 0) return hipSuccess;
     TRY(by_shape(P, [&](auto sh) {
         hipLaunchKernelGGL(enc_front_kernel<decltype(sh)>, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk,
-                           nullptr, W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, (int64_t)0);
+                           nullptr, W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, (int64_t)0, nullptr);
         return hipGetLastError();
     }));
     return enc_finish(P, W, backend, ct, ss, B, status, st);
@@ -1117,18 +1124,21 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
                             hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
-    // the MFMA product reads A from the key store at the key id; the VALU
-    // backend gets a gathered copy
-    const int abytes = backend == 0 ? 0 : P.l * P.l * kN * 2;
-    TRY(gather(GatherSet{{K.pk, K.hpk, reinterpret_cast<const uint8_t *>(K.A)},
-                         {P.pk, 32, abytes},
-                         {W.pkg, W.hpkg, reinterpret_cast<uint8_t *>(W.A)},
-                         {P.pk, 32, abytes},
-                         {P.pk, 32, abytes}},
-               keys, B, st));
+    // The front kernel reads pk and H(pk) from the key store at the key id,
+    // and the MFMA product reads A there too; the VALU backend gets a
+    // gathered copy of A.
+    if (backend != 0) {
+        const int abytes = P.l * P.l * kN * 2;
+        TRY(gather(GatherSet{{reinterpret_cast<const uint8_t *>(K.A), nullptr, nullptr},
+                             {abytes, 0, 0},
+                             {reinterpret_cast<uint8_t *>(W.A), nullptr, nullptr},
+                             {abytes, 0, 0},
+                             {abytes, 0, 0}},
+                   keys, B, st));
+    }
     TRY(by_shape(P, [&](auto sh) {
-        hipLaunchKernelGGL(enc_front_kernel<decltype(sh)>, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, W.pkg, W.hpkg,
-                           W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, B);
+        hipLaunchKernelGGL(enc_front_kernel<decltype(sh)>, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, K.pk, K.hpk,
+                           W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, B, keys);
         return hipGetLastError();
     }));
     if (backend == 0) return enc_finish(P, W, backend, ct, ss, B, status, st, K.A, keys);
@@ -1178,11 +1188,9 @@ size_t kem_workspace_bytes(const KemParams &P, int64_t B)
     b += n * 64;             // buf
     b += n * 64;             // kr
     b += n * 4;              // diff
-    b += n * (size_t)P.pk;   // pkg
     b += n * (size_t)P.sk;   // skg
-    b += n * 32;             // hpkg
     b += n * 32;             // hct
-    return b + 15 * 256;
+    return b + 11 * 256;     // each of the 11 pieces (kem_workspace_carve) rounded up to 256 bytes
 }
 
 void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorkspace &W)
@@ -1202,9 +1210,7 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
     W.buf = take(n * 64);
     W.kr = take(n * 64);
     W.diff = reinterpret_cast<uint32_t *>(take(n * 4));
-    W.pkg = take(n * (size_t)P.pk);
     W.skg = take(n * (size_t)P.sk);
-    W.hpkg = take(n * 32);
     W.hct = take(n * 32);
 }
 
