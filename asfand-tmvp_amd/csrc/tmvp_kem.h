@@ -23,7 +23,6 @@ struct KemWorkspace {
     uint16_t *A, *s, *v1, *v2, *bits, *poly;
     uint8_t *buf, *kr;
     uint32_t *diff; // decapsulation: re-encryption differs from ct
-    uint8_t *skg;              // secret keys gathered from a key store
     uint8_t *hct;              // H(ct) per instance (decapsulation)
     // Decapsulation runs the steps that depend only on its inputs (the matrix
     // stream, H(ct)) on `aux`, forked from and joined to the caller's stream

@@ -429,7 +429,10 @@ __global__ __launch_bounds__(64) void kp_front_kernel(KemParams P, const uint8_t
     else kp_secret_role<SH, 1>(P, seeds, sk, S, t - B, ring, p);
 }
 
-template <class SH>
+// KEYED (A resident, t0 = B): pk and hpk are key stores, read at keys[i].  A
+// template flag rather than a null test of keys: with the runtime select the
+// non-keyed Saber kernel measured 176 us at B = 16384 against 159 us without.
+template <class SH, bool KEYED>
 __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_t *rnd, const uint8_t *pk,
                                                        const uint8_t *hpk, uint8_t *buf, uint8_t *kr, uint16_t *A,
                                                        uint16_t *S, uint16_t *V, uint16_t *bits,
@@ -441,9 +444,8 @@ __global__ __launch_bounds__(64) void enc_front_kernel(KemParams P, const uint8_
     if (t < B) {
         enc_matrix_role<SH, 2>(P, pk, A, t, ring, p);
     } else {
-        // keys (A resident, t0 = B): pk and hpk are key stores, read at the key id
         const int64_t i = t - B;
-        enc_secret_role<SH, 2>(P, rnd, pk, hpk, buf, kr, S, V, bits, i, keys ? (int64_t)keys[i] : i, ring, p);
+        enc_secret_role<SH, 2>(P, rnd, pk, hpk, buf, kr, S, V, bits, i, KEYED ? (int64_t)keys[i] : i, ring, p);
     }
 }
 
@@ -474,11 +476,12 @@ __global__ __launch_bounds__(64) void dec_pre_kernel(KemParams P, const uint8_t 
 // V for the re-encryption, and the instance's mismatch flag cleared (pairs
 // [0, B)).  With hct set, pairs [B, 2B) compute H(ct) beside it (the
 // decapsulation of keys whose A is resident has no matrix sponge to run it
-// next to).
+// next to).  With skey set, instance t's sk is row skey[t] of sk.
 template <class SH>
 __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16_t *mbits, const uint8_t *sk,
                                                        uint8_t *buf, uint8_t *kr, uint16_t *S, uint16_t *V,
-                                                       uint32_t *diff, int64_t B, const uint8_t *ct, uint8_t *hct)
+                                                       uint32_t *diff, int64_t B, const uint8_t *ct, uint8_t *hct,
+                                                       const int32_t *skey)
 {
     TMVP_PAIR(0);
     TMVP_RING;
@@ -505,10 +508,11 @@ __global__ __launch_bounds__(64) void dec_front_kernel(KemParams P, const uint16
         m[w] = x;
     }
     uint32_t h[4], coins[4];
-    load_words(h, sk + t * P.sk + P.sk - 64, p);
+    const uint8_t *skt = sk + (skey ? (int64_t)skey[t] : t) * P.sk;
+    load_words(h, skt + P.sk - 64, p);
     hash_g<1>(m, h, buf + t * 64, kr + t * 64, coins, p);
     secret_stream<SH::MU, 1>(P, coins, S + t * P.l * kN, ring, p);
-    unpack_b<SH::EP>(P, sk + t * P.sk + P.indcpa_sk, V + t * P.l * kN, p);
+    unpack_b<SH::EP>(P, skt + P.indcpa_sk, V + t * P.l * kN, p);
 }
 
 // ---------------------------------------------------------------- codecs
@@ -639,10 +643,11 @@ __global__ __launch_bounds__(256) void unpack_kernel(const uint8_t *src, int64_t
 // bits, or the Sable 2-bit code), 4L chunks of b' (WB bits) and 4 chunks of
 // c_m (WC bits) from the ct.  Used where nothing runs beside the decryption
 // (A resident); beside the matrix sponge, three launches measured faster.
+// With skey set, instance t's sk is row skey[t] of sk (a key store).
 template <int WS, int XS, int WB, int WC>
 __global__ __launch_bounds__(256) void dec_unpack_kernel(const uint8_t *sk, int64_t sk_stride, const uint8_t *ct,
                                                          int64_t ct_stride, int pvc, uint16_t *S, uint16_t *Bp,
-                                                         uint16_t *Cm, int l, int64_t B)
+                                                         uint16_t *Cm, int l, int64_t B, const int32_t *skey)
 {
     const int cv = 4 * l; // chunks of one polynomial vector
     const int chunks = 2 * cv + 4;
@@ -651,7 +656,8 @@ __global__ __launch_bounds__(256) void dec_unpack_kernel(const uint8_t *sk, int6
     const int64_t t = gid / chunks;
     const int c = (int)(gid - t * chunks);
     if (c < cv)
-        unpack_chunk<WS, XS>(reinterpret_cast<const uint64_t *>(sk + t * sk_stride + (int64_t)c * WS * 8),
+        unpack_chunk<WS, XS>(reinterpret_cast<const uint64_t *>(sk + (skey ? (int64_t)skey[t] : t) * sk_stride +
+                                                                (int64_t)c * WS * 8),
                              reinterpret_cast<uint4 *>(S + (t * cv + c) * 64));
     else if (c < 2 * cv)
         unpack_chunk<WB, kXfNone>(reinterpret_cast<const uint64_t *>(ct + t * ct_stride + (int64_t)(c - cv) * WB * 8),
@@ -772,7 +778,7 @@ __global__ __launch_bounds__(64) void enc_hash2_kernel(KemParams P, uint8_t *ct,
 // records the outcome.
 __global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint32_t *diff, const uint8_t *sk,
                                                      const uint8_t *hct, const uint8_t *kr, uint8_t *ss, uint8_t *fail,
-                                                     int64_t B)
+                                                     int64_t B, const int32_t *skey)
 {
     TMVP_PAIR(0);
     if (t >= B) return;
@@ -780,7 +786,7 @@ __global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint32_t
     uint32_t h[4], k[8], s[4], k0[4], z[4];
     load_words(h, hct + t * 32, p);
     load_words(k0, kr + t * 64, p);
-    load_words(z, sk + t * P.sk + P.sk - 32, p);
+    load_words(z, sk + (skey ? (int64_t)skey[t] : t) * P.sk + P.sk - 32, p);
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         k[i] = k0[i] ^ (mask & (k0[i] ^ z[i]));
@@ -879,10 +885,10 @@ static hipError_t unpack(const uint8_t *src, int64_t src_stride, int src_off, ui
 
 template <int WS, int XS, int WB, int WC>
 static hipError_t dec_unpack_w(const KemParams &P, const uint8_t *sk, const uint8_t *ct, uint16_t *S, uint16_t *Bp,
-                               uint16_t *Cm, int64_t B, hipStream_t st)
+                               uint16_t *Cm, int64_t B, hipStream_t st, const int32_t *skey)
 {
     hipLaunchKernelGGL((dec_unpack_kernel<WS, XS, WB, WC>), dim3(blocks_for(B * (8 * P.l + 4), 256)), dim3(256), 0,
-                       st, sk, (int64_t)P.sk, ct, (int64_t)P.ct, P.pvc, S, Bp, Cm, P.l, B);
+                       st, sk, (int64_t)P.sk, ct, (int64_t)P.ct, P.pvc, S, Bp, Cm, P.l, B, skey);
     return hipGetLastError();
 }
 
@@ -910,17 +916,17 @@ static hipError_t dec_compare(const KemParams &P, const uint16_t *bp, const uint
 // s (from the sk), b' and c_m (from the ct) of a decapsulation in one launch,
 // for the six parameter sets' widths.
 static hipError_t dec_unpack(const KemParams &P, const uint8_t *sk, const uint8_t *ct, uint16_t *S, uint16_t *Bp,
-                             uint16_t *Cm, int64_t B, hipStream_t st)
+                             uint16_t *Cm, int64_t B, hipStream_t st, const int32_t *skey)
 {
     if (!P.sable && P.eq == 13 && P.ep == 10) {
-        if (P.et == 3) return dec_unpack_w<13, kXfNone, 10, 3>(P, sk, ct, S, Bp, Cm, B, st);
-        if (P.et == 4) return dec_unpack_w<13, kXfNone, 10, 4>(P, sk, ct, S, Bp, Cm, B, st);
-        if (P.et == 6) return dec_unpack_w<13, kXfNone, 10, 6>(P, sk, ct, S, Bp, Cm, B, st);
+        if (P.et == 3) return dec_unpack_w<13, kXfNone, 10, 3>(P, sk, ct, S, Bp, Cm, B, st, skey);
+        if (P.et == 4) return dec_unpack_w<13, kXfNone, 10, 4>(P, sk, ct, S, Bp, Cm, B, st, skey);
+        if (P.et == 6) return dec_unpack_w<13, kXfNone, 10, 6>(P, sk, ct, S, Bp, Cm, B, st, skey);
     }
     if (P.sable) {
-        if (P.ep == 9 && P.et == 3) return dec_unpack_w<2, kXfSk2, 9, 3>(P, sk, ct, S, Bp, Cm, B, st);
-        if (P.ep == 9 && P.et == 5) return dec_unpack_w<2, kXfSk2, 9, 5>(P, sk, ct, S, Bp, Cm, B, st);
-        if (P.ep == 10 && P.et == 3) return dec_unpack_w<2, kXfSk2, 10, 3>(P, sk, ct, S, Bp, Cm, B, st);
+        if (P.ep == 9 && P.et == 3) return dec_unpack_w<2, kXfSk2, 9, 3>(P, sk, ct, S, Bp, Cm, B, st, skey);
+        if (P.ep == 9 && P.et == 5) return dec_unpack_w<2, kXfSk2, 9, 5>(P, sk, ct, S, Bp, Cm, B, st, skey);
+        if (P.ep == 10 && P.et == 3) return dec_unpack_w<2, kXfSk2, 10, 3>(P, sk, ct, S, Bp, Cm, B, st, skey);
     }
     return hipErrorInvalidValue;
 }
@@ -1032,7 +1038,7 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
 {
     if (B <= 0) return hipSuccess;
     TRY(by_shape(P, [&](auto sh) {
-        hipLaunchKernelGGL(enc_front_kernel<decltype(sh)>, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk,
+        hipLaunchKernelGGL((enc_front_kernel<decltype(sh), false>), dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk,
                            nullptr, W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, (int64_t)0, nullptr);
         return hipGetLastError();
     }));
@@ -1044,10 +1050,12 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
 // unpacking) and H(ct) depend only on the inputs and run on W.aux beside the
 // decryption and the InnerProd half of the re-encryption.
 // resident: A is already in place (W.A, or the key store A at akey[i] for the
-// MFMA backend), so the matrix sponge is skipped.
+// MFMA backend), so the matrix sponge is skipped; skey (resident only): instance
+// i's sk is row skey[i] of sk (a key store).
 static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *ct,
                                const uint8_t *sk, uint8_t *ss, uint8_t *fail, int64_t B, int *status, hipStream_t st,
-                               bool resident, const uint16_t *A = nullptr, const int32_t *akey = nullptr)
+                               bool resident, const uint16_t *A = nullptr, const int32_t *akey = nullptr,
+                               const int32_t *skey = nullptr)
 {
     const int L = P.l;
     // Without resident A, the matrix sponge and H(ct) (dec_pre_kernel) run on
@@ -1065,7 +1073,7 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     }
     // IND-CPA decryption (S/src/cuda_kernel.cu:341-370; SB/src/SABLE_indcpa.cu:422-463)
     if (resident) {
-        TRY(dec_unpack(P, sk, ct, W.s, W.v2, W.poly, B, st));
+        TRY(dec_unpack(P, sk, ct, W.s, W.v2, W.poly, B, st, skey));
     } else {
         if (P.sable)
             TRY(unpack(sk, P.sk, 0, W.s, L * kN, 2, L * kN, kXfSk2, B, st));
@@ -1079,7 +1087,7 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
     // re-encryption of m' under the pk stored in the sk
     TRY(by_shape(P, [&](auto sh) {
         hipLaunchKernelGGL(dec_front_kernel<decltype(sh)>, dim3(pair_blocks(resident ? 2 * B : B)), dim3(64), 0, st, P,
-                           W.bits, sk, W.buf, W.kr, W.s, W.v2, W.diff, B, ct, resident ? W.hct : nullptr);
+                           W.bits, sk, W.buf, W.kr, W.s, W.v2, W.diff, B, ct, resident ? W.hct : nullptr, skey);
         return hipGetLastError();
     }));
     // ct is only read: the compare-packs never write through it
@@ -1099,7 +1107,7 @@ static hipError_t decaps_chain(const KemParams &P, const KemWorkspace &W, int ba
         TRY(enc_matvec_half(P, W, backend, ctc, W.diff, B, status, st));
     }
     hipLaunchKernelGGL(dec_fin_kernel, dim3(pair_blocks(B)), dim3(64), 0, st, P, W.diff, sk, W.hct, W.kr, ss, fail,
-                       B);
+                       B, skey);
     return hipGetLastError();
 }
 
@@ -1137,7 +1145,7 @@ hipError_t kem_encaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
                    keys, B, st));
     }
     TRY(by_shape(P, [&](auto sh) {
-        hipLaunchKernelGGL(enc_front_kernel<decltype(sh)>, dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, K.pk, K.hpk,
+        hipLaunchKernelGGL((enc_front_kernel<decltype(sh), true>), dim3(pair_blocks(B)), dim3(64), 0, st, P, rnd, K.pk, K.hpk,
                            W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, B, keys);
         return hipGetLastError();
     }));
@@ -1150,16 +1158,20 @@ hipError_t kem_decaps_keyed(const KemParams &P, const KemWorkspace &W, int backe
                             int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
-    // as in kem_encaps_keyed: A from the key store for the MFMA product
-    const int abytes = backend == 0 ? 0 : P.l * P.l * kN * 2;
-    TRY(gather(GatherSet{{K.sk, reinterpret_cast<const uint8_t *>(K.A), nullptr},
-                         {P.sk, abytes, 0},
-                         {W.skg, reinterpret_cast<uint8_t *>(W.A), nullptr},
-                         {P.sk, abytes, 0},
-                         {P.sk, abytes, 0}},
-               keys, B, st));
-    if (backend == 0) return decaps_chain(P, W, backend, ct, W.skg, ss, fail, B, status, st, true, K.A, keys);
-    return decaps_chain(P, W, backend, ct, W.skg, ss, fail, B, status, st, true);
+    // The decode, front and final kernels read the sk from the key store at
+    // the key id; as in kem_encaps_keyed, the MFMA product reads A there too
+    // and the VALU backend gets a gathered copy of A.
+    if (backend != 0) {
+        const int abytes = P.l * P.l * kN * 2;
+        TRY(gather(GatherSet{{reinterpret_cast<const uint8_t *>(K.A), nullptr, nullptr},
+                             {abytes, 0, 0},
+                             {reinterpret_cast<uint8_t *>(W.A), nullptr, nullptr},
+                             {abytes, 0, 0},
+                             {abytes, 0, 0}},
+                   keys, B, st));
+        return decaps_chain(P, W, backend, ct, K.sk, ss, fail, B, status, st, true, nullptr, nullptr, keys);
+    }
+    return decaps_chain(P, W, backend, ct, K.sk, ss, fail, B, status, st, true, K.A, keys, keys);
 }
 
 hipError_t kem_store_derive(const KemParams &P, const uint8_t *pk, int64_t n, uint16_t *A, uint8_t *hpk,
@@ -1188,9 +1200,8 @@ size_t kem_workspace_bytes(const KemParams &P, int64_t B)
     b += n * 64;             // buf
     b += n * 64;             // kr
     b += n * 4;              // diff
-    b += n * (size_t)P.sk;   // skg
     b += n * 32;             // hct
-    return b + 11 * 256;     // each of the 11 pieces (kem_workspace_carve) rounded up to 256 bytes
+    return b + 10 * 256;     // each of the 10 pieces (kem_workspace_carve) rounded up to 256 bytes
 }
 
 void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorkspace &W)
@@ -1210,7 +1221,6 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
     W.buf = take(n * 64);
     W.kr = take(n * 64);
     W.diff = reinterpret_cast<uint32_t *>(take(n * 4));
-    W.skg = take(n * (size_t)P.sk);
     W.hct = take(n * 32);
 }
 

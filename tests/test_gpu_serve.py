@@ -259,12 +259,14 @@ def test_serving_pinned_input_alignment(ctx, keys, offset):
             assert np.array_equal(got, exp), (k, op, ps, offset)
 
 
-def test_serving_all_parameter_sets(ctx):
+@pytest.mark.parametrize("backend", [T.MFMA, T.VALU], ids=["mfma", "valu"])
+def test_serving_all_parameter_sets(ctx, backend):
     """Keyed encapsulation and decapsulation through the server for every
-    parameter set give the plain batched calls' bytes."""
+    parameter set, on either product backend (the MFMA one reads A at the key
+    id, the VALU one a gathered copy), give the plain batched calls' bytes."""
     rng = np.random.default_rng(77)
     sets = ("lightsaber", "saber", "firesaber", "sable1", "sable2", "sable3")
-    srv = T.Server(ctx, max_batch=1024, slots=2)
+    srv = T.Server(ctx, max_batch=1024, slots=2, backend=backend)
     try:
         want, outs = {}, {}
         for i, ps in enumerate(sets):
