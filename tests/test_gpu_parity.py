@@ -68,9 +68,14 @@ def test_polymul_unreduced_public_and_closed_forms(ctx, backend):
     got = T.run_polymul(ctx, a, s, 13, backend)
     want = O.polymul_batch(a, s) & 8191
     assert np.array_equal(got, want)
-    k = 0
-    rot = np.concatenate([(-a[0, N - k:].astype(np.int64)) % 8192, a[0, :N - k] % 8192]) if k else a[0] % 8192
-    assert np.array_equal(got[0], rot.astype(np.uint16))
+    # closed form: a * x^k is a rotated by k with the k wrapped coefficients
+    # negated (x^256 = -1); -x^k negates the whole rotation
+    for i in range(16):
+        k = (17 * i) % N
+        rot = np.concatenate([-a[i, N - k:].astype(np.int64), a[i, :N - k].astype(np.int64)])
+        if i % 2:
+            rot = -rot
+        assert np.array_equal(got[i], (rot % 8192).astype(np.uint16)), f"instance {i}, k = {k}"
 
 
 @pytest.mark.parametrize("eq_bits", [8, 11, 14])
@@ -134,6 +139,44 @@ def test_innerprod_vs_oracle(ctx, backend, ps):
     got = T.run_innerprod(ctx, b, s, ps, backend, T.IP_DEC, aux=cm)
     want = ((raw + p["h2"] - (cm.astype(np.int64) << (ep - et))) % (1 << ep)) >> (ep - 1)
     assert np.array_equal(got, want.astype(np.uint16))
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("ps", PS_ALL)
+def test_enc_dec_epilogues_decoding_window_on_gpu(ctx, backend, ps):
+    """The fused encryption / decryption roundings of tmvp_innerprod over every
+    v in Z_p and m in {0, 1}: bit-exact against the oracle's post_process2 /
+    post_process3 forms (S/src/poly.cu:391-403; SB/src/pack.cu:290-335, 436-442),
+    and, from the mathematics, decoding recovers m for every noise |e| <= W =
+    2^(EP-2) - 2^(EP-ET-1) and fails for some v at e = +-(W + 1).  The secret is
+    the polynomial 1, so the InnerProd passes b through to the epilogue."""
+    p = O.params(ps)
+    l, ep, et = p["l"], p["ep"], p["et"]
+    P, W = 1 << ep, (1 << (ep - 2)) - (1 << (ep - et - 1))
+    vals = np.concatenate([np.arange(P), np.arange(P)])
+    ms = np.concatenate([np.zeros(P, np.int64), np.ones(P, np.int64)])
+
+    def through_epilogue(x, aux, epilogue):
+        x = np.asarray(x).reshape(-1, N)
+        batch = x.shape[0]
+        b = np.zeros((batch, l * N), np.uint16)
+        b[:, :N] = x
+        s = np.zeros((batch, l * N), np.uint16)
+        s[:, 0] = 1
+        got = T.run_innerprod(ctx, b, s, ps, backend, epilogue, aux=np.asarray(aux, np.uint16).reshape(batch, N))
+        return got.reshape(-1).astype(np.int64)
+
+    c = through_epilogue(vals, ms, T.IP_ENC)
+    assert np.array_equal(c, O.round_enc_v(ps, vals, ms).astype(np.int64))
+    es = np.arange(-W - 1, W + 2)
+    x = (vals[None, :] + es[:, None]) % P
+    aux = np.broadcast_to(c, x.shape)
+    m = through_epilogue(x, aux, T.IP_DEC).reshape(x.shape)
+    dec_table = np.stack([O.round_dec_m(ps, np.arange(P), np.full(P, cc)) for cc in range(1 << et)]).astype(np.int64)
+    assert np.array_equal(m, dec_table[aux, x])
+    fails = np.count_nonzero(m != ms[None, :], axis=1)
+    assert not fails[1:-1].any(), f"decoding fails inside the window at e = {es[1:-1][fails[1:-1] > 0]}"
+    assert fails[0] > 0 and fails[-1] > 0
 
 
 # ---------------------------------------------------------------- reference test vectors

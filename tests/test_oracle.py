@@ -209,6 +209,93 @@ def test_rounding_family_forms_equal_closed_form(ps):
     assert np.array_equal(m, ((x + h2 - (c << (ep - et))) % (1 << ep)) >> (ep - 1))
 
 
+ALL_SETS = ["lightsaber", "saber", "firesaber", "sable1", "sable2", "sable3"]
+# Decryption noise tolerance |e| <= 2^(EP-2) - 2^(EP-ET-1) per set (derived below).
+DECODE_WINDOW = {"lightsaber": 192, "saber": 224, "firesaber": 248, "sable1": 96, "sable2": 120, "sable3": 192}
+
+
+def rounding_tables(ps):
+    """enc[m][v] = or_round_enc_v(v, m) for v in Z_p, and dec[c][x] = or_round_dec_m(x, c)
+    for every x in Z_p and every wire value c in [0, 2^ET)."""
+    p = O.params(ps)
+    P, T = 1 << p["ep"], 1 << p["et"]
+    v = np.arange(P)
+    enc = np.stack([O.round_enc_v(ps, v, np.full(P, m)) for m in (0, 1)]).astype(np.int64)
+    dec = np.stack([O.round_dec_m(ps, v, np.full(P, c)) for c in range(T)]).astype(np.int64)
+    return enc, dec
+
+
+def decode_failures(ps, enc, dec, e):
+    """Messages bits that decode wrongly when decryption sees v + e instead of v,
+    over every v in Z_p and m in {0, 1}."""
+    P = 1 << O.params(ps)["ep"]
+    v = np.arange(P)
+    bad = 0
+    for m in (0, 1):
+        c = enc[m]
+        got = dec[c, (v + e) % P]
+        bad += int(np.count_nonzero(got != m))
+    return bad
+
+
+@pytest.mark.parametrize("ps", ALL_SETS)
+def test_encaps_decaps_rounding_decoding_window(ps):
+    """Pins the encryption rounding (post_process2, S/src/poly.cu:391-395; SB/src/pack.cu:290-335)
+    and the decryption rounding (post_process3, S/src/poly.cu:398-403; SB/src/pack.cu:436-442)
+    from what they are for, not from their formula.
+
+    Encryption sends c = floor((v - m 2^(EP-1) + h1) / 2^(EP-ET)) mod 2^ET, so
+    c 2^(EP-ET) = v - m 2^(EP-1) + h1 - r with r in [0, 2^(EP-ET)) depending on v.
+    Decryption sees v + e and computes x = v + e + h2 - c 2^(EP-ET)
+    = m 2^(EP-1) + e + (h2 - h1) + r (mod p), and m is recovered exactly when
+    0 <= e + (h2 - h1) + r < 2^(EP-1) for every r.  With h2 - h1 = 2^(EP-2) - 2^(EP-ET-1)
+    that is the symmetric window |e| <= W = 2^(EP-2) - 2^(EP-ET-1): every (v, m)
+    decodes for |e| <= W and some (v, m) fails at e = W + 1 and at e = -(W + 1).
+    A dropped h1, an h2 off by a step, a wrong shift or a sign slip in either
+    rounding moves or narrows the window and fails here."""
+    p = O.params(ps)
+    W = (1 << (p["ep"] - 2)) - (1 << (p["ep"] - p["et"] - 1))
+    assert W == DECODE_WINDOW[ps]
+    enc, dec = rounding_tables(ps)
+    assert enc.min() >= 0 and enc.max() < (1 << p["et"])
+    assert set(np.unique(dec)) <= {0, 1}
+    for e in range(-W, W + 1):
+        assert decode_failures(ps, enc, dec, e) == 0, f"{ps}: noise {e} inside the window fails"
+    assert decode_failures(ps, enc, dec, W + 1) > 0, f"{ps}: window wider than W on the + side"
+    assert decode_failures(ps, enc, dec, -(W + 1)) > 0, f"{ps}: window wider than W on the - side"
+
+
+def test_decoding_window_catches_planted_rounding_slips():
+    """The window check above rejects the plausible slips: h1 dropped from the
+    encryption rounding, h2 one half-step off, and the two message halves swapped."""
+    ps = "saber"
+    p = O.params(ps)
+    P, T = 1 << p["ep"], 1 << p["et"]
+    ep, et, h1, h2 = p["ep"], p["et"], p["h1"], p["h2"]
+    v = np.arange(P)
+    W = DECODE_WINDOW[ps]
+
+    def window_ok(enc_f, dec_f):
+        enc = np.stack([enc_f(v, m) for m in (0, 1)])
+        dec = np.stack([dec_f(v, c) for c in range(T)])
+        inside = all(decode_failures(ps, enc, dec, e) == 0 for e in range(-W, W + 1))
+        edges = decode_failures(ps, enc, dec, W + 1) > 0 and decode_failures(ps, enc, dec, -W - 1) > 0
+        return inside and edges
+
+    def enc_ok(x, m, hh=h1):
+        return ((x - (m << (ep - 1)) + hh) % P) >> (ep - et)
+
+    def dec_ok(x, c, hh=h2):
+        return ((x + hh - (c << (ep - et))) % P) >> (ep - 1)
+
+    enc_tab, dec_tab = rounding_tables(ps)
+    assert window_ok(lambda x, m: enc_tab[m][x], lambda x, c: dec_tab[c][x])   # the oracle itself
+    assert not window_ok(lambda x, m: enc_ok(x, m, 0), dec_ok)                 # h1 dropped
+    assert not window_ok(enc_ok, lambda x, c: dec_ok(x, c, h2 + (1 << (ep - et - 1))))
+    assert not window_ok(enc_ok, lambda x, c: dec_ok(x, c, h2 - (1 << (ep - et - 1))))
+    assert not window_ok(lambda x, m: enc_ok(x, 1 - m), dec_ok)                # message halves swapped
+
+
 def test_params_table_matches_headers():
     # byte sizes in S/include/params.h and SB/include/SABLE_params.h (Saber L=3, Sable type 2)
     s = O.params("saber")
