@@ -12,10 +12,29 @@ using namespace tmvp;
 
 namespace {
 constexpr int kMaxWorkspaces = 16;
-struct Workspace {
-    hipStream_t stream;
+// Device scratch of a workspace.  A graph captured on the workspace's stream
+// holds a reference: its kernels point into the buffer, so the workspace
+// never frees or hands out a buffer that a live graph still holds, and moves
+// to a new one instead when it has to grow or serve another stream.
+struct Buf {
     void *ptr;
     size_t bytes;
+    int refs;
+    int device;
+};
+void buf_release(Buf *b)
+{
+    if (!b || --b->refs > 0) return;
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != b->device) (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize(); // a replay may still run on some stream
+    (void)hipFree(b->ptr);
+    if (cur >= 0 && cur != b->device) (void)hipSetDevice(cur);
+    delete b;
+}
+struct Workspace {
+    hipStream_t stream;
+    Buf *buf;          // null until the first KEM call on the stream
     uint64_t used;     // last-use stamp
     hipStream_t aux;   // side stream for decapsulation steps that depend only on inputs
     hipEvent_t fork, join;
@@ -43,6 +62,7 @@ struct tmvp_ctx {
 struct tmvp_graph {
     hipGraph_t graph;
     hipGraphExec_t exec;
+    Buf *ws; // the workspace buffer the captured KEM kernels use, or null
 };
 
 namespace {
@@ -131,8 +151,7 @@ Workspace *workspace(tmvp_ctx *ctx, hipStream_t stream, size_t bytes, int *rc)
         }
         if (ctx->nws < kMaxWorkspaces) {
             w = &ctx->ws[ctx->nws];
-            w->ptr = nullptr;
-            w->bytes = 0;
+            w->buf = nullptr;
             if (hipStreamCreateWithFlags(&w->aux, hipStreamNonBlocking) != hipSuccess) {
                 *rc = TMVP_EHIP;
                 return nullptr;
@@ -152,11 +171,16 @@ Workspace *workspace(tmvp_ctx *ctx, hipStream_t stream, size_t bytes, int *rc)
                 *rc = TMVP_EHIP;
                 return nullptr;
             }
+            // the evicted stream's buffer is reused unless a graph holds it
+            if (w->buf && w->buf->refs > 1) {
+                buf_release(w->buf);
+                w->buf = nullptr;
+            }
         }
         w->stream = stream;
     }
     w->used = ++ctx->ws_clock;
-    if (w->bytes < bytes) {
+    if (!w->buf || w->buf->bytes < bytes) {
         if (capturing(stream)) { // no allocation inside a graph capture
             *rc = TMVP_EUNSUPPORTED;
             return nullptr;
@@ -165,14 +189,19 @@ Workspace *workspace(tmvp_ctx *ctx, hipStream_t stream, size_t bytes, int *rc)
             *rc = TMVP_EHIP;
             return nullptr;
         }
-        if (w->ptr) (void)hipFree(w->ptr);
-        w->ptr = nullptr;
-        w->bytes = 0;
-        if (hipMalloc(&w->ptr, bytes) != hipSuccess) {
+        buf_release(w->buf); // freed here unless a graph still holds it
+        w->buf = nullptr;
+        Buf *b = new (std::nothrow) Buf{nullptr, bytes, 1, ctx->device};
+        if (!b) {
             *rc = TMVP_ENOMEM;
             return nullptr;
         }
-        w->bytes = bytes;
+        if (hipMalloc(&b->ptr, bytes) != hipSuccess) {
+            delete b;
+            *rc = TMVP_ENOMEM;
+            return nullptr;
+        }
+        w->buf = b;
     }
     return w;
 }
@@ -190,7 +219,7 @@ int kem_setup(tmvp_ctx *ctx, int ps, int64_t batch, tmvp_stream_t stream, KemPar
     P = kem_params(ps);
     Workspace *w = workspace(ctx, reinterpret_cast<hipStream_t>(stream), kem_workspace_bytes(P, batch), &rc);
     if (!w) return rc;
-    kem_workspace_carve(P, batch, static_cast<uint8_t *>(w->ptr), W);
+    kem_workspace_carve(P, batch, static_cast<uint8_t *>(w->buf->ptr), W);
     W.aux = w->aux;
     W.fork = w->fork;
     W.join = w->join;
@@ -454,7 +483,7 @@ int tmvp_destroy(tmvp_ctx *ctx)
     int rc = set_device(ctx);
     if (hipDeviceSynchronize() != hipSuccess) rc = TMVP_EHIP;
     for (int i = 0; i < ctx->nws; i++) {
-        if (ctx->ws[i].ptr) (void)hipFree(ctx->ws[i].ptr);
+        buf_release(ctx->ws[i].buf); // graphs still alive keep theirs
         (void)hipEventDestroy(ctx->ws[i].fork);
         (void)hipEventDestroy(ctx->ws[i].join);
         (void)hipStreamDestroy(ctx->ws[i].aux);
@@ -728,6 +757,15 @@ int tmvp_graph_end(tmvp_ctx *ctx, tmvp_stream_t stream, tmvp_graph **out)
     }
     h->graph = g;
     h->exec = x;
+    h->ws = nullptr;
+    // The captured KEM kernels point into the workspace of this stream: the
+    // graph keeps that buffer alive until it is destroyed.
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    for (int i = 0; i < ctx->nws; i++)
+        if (ctx->ws[i].stream == st && ctx->ws[i].buf) {
+            h->ws = ctx->ws[i].buf;
+            h->ws->refs++;
+        }
     *out = h;
     return TMVP_OK;
 }
@@ -744,6 +782,7 @@ int tmvp_graph_destroy(tmvp_graph *graph)
     int rc = TMVP_OK;
     if (hipGraphExecDestroy(graph->exec) != hipSuccess) rc = TMVP_EHIP;
     if (hipGraphDestroy(graph->graph) != hipSuccess) rc = TMVP_EHIP;
+    buf_release(graph->ws);
     delete graph;
     return rc;
 }

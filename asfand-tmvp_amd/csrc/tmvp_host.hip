@@ -126,7 +126,15 @@ int host_enc(HostState &S, int ps, int64_t batch, int mode, uint8_t *h_k, const 
         return rc;
     uint8_t *rnd = S.rnd;
     uint8_t *fresh = nullptr;
-    if (!rnd || S.rnd_bytes != B * 32) {
+    if (rnd && S.rnd_bytes != B * 32) {
+        // randomness set for another batch size: an error, not a silent
+        // switch to fresh randomness (the caller expects its own coins)
+        delete[] S.rnd;
+        S.rnd = nullptr;
+        S.rnd_bytes = 0;
+        return TMVP_EARG;
+    }
+    if (!rnd) {
         fresh = new uint8_t[B * 32];
         if ((rc = fill_random(fresh, B * 32))) {
             delete[] fresh;

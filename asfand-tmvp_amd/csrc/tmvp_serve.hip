@@ -59,7 +59,8 @@ struct Store {
 
 struct Pending {
     uint64_t id;
-    tmvp_srv_request req;
+    tmvp_srv_request req;       // req.keys is not read after submit:
+    std::vector<int32_t> keys;  // the ids are copied here as they are checked
     double t_submit;
     bool pinned_in, pinned_out; // host buffers the device can reach directly
     const uint8_t *dev_in;      // pinned_in: the device's address of req.in
@@ -249,7 +250,7 @@ int launch_slot(tmvp_server *S, Slot &s)
     };
     int64_t off = 0;
     for (const Pending &p : s.reqs) {
-        memcpy(hkeys + off, p.req.keys, (size_t)p.req.count * 4);
+        memcpy(hkeys + off, p.keys.data(), (size_t)p.req.count * 4);
         off += p.req.count;
     }
     bool fits = add(s.h_in_dev, s.d_in, (size_t)B * 4);
@@ -389,7 +390,7 @@ bool pick_batch(tmvp_server *S, Slot &s)
     s.reqs.clear();
     while (!q.empty() && s.instances + q.front().req.count <= S->max_batch) {
         s.instances += q.front().req.count;
-        s.reqs.push_back(q.front());
+        s.reqs.push_back(std::move(q.front()));
         q.pop_front();
     }
     S->pending_requests -= (int64_t)s.reqs.size();
@@ -570,6 +571,12 @@ int tmvp_srv_add_keys(tmvp_server *S, int ps, const uint8_t *h_pk, const uint8_t
     if (first_id) *first_id = st.n;
     if (n == 0) return TMVP_OK;
     const KemParams P = kem_params_of(ps);
+    // Keyed decapsulation reads A and H(pk) derived from h_pk, and b, H(pk)
+    // and z from the sk: each sk must hold its own pk (sk = s | pk | H(pk) | z).
+    if (h_sk)
+        for (int64_t i = 0; i < n; i++)
+            if (memcmp(h_sk + (size_t)i * P.sk + P.indcpa_sk, h_pk + (size_t)i * P.pk, (size_t)P.pk) != 0)
+                return TMVP_EARG;
     if (st.n == 0) st.has_sk = h_sk != nullptr;
     int rc = grow_store(st, P, st.n + n, st.has_sk);
     if (rc) return rc;
@@ -597,15 +604,22 @@ int tmvp_srv_submit(tmvp_server *S, const tmvp_srv_request *r, uint64_t *id)
     const bool pinned_in = is_pinned(r->in, &dev_in);
     const bool pinned_out = is_pinned(r->ss, &dev_ss) && (r->op != TMVP_SRV_ENCAPS || is_pinned(r->ct, &dev_ct)) &&
                             (r->op != TMVP_SRV_DECAPS || is_pinned(r->fail, &dev_fail));
+    if (r->count > S->max_batch) return TMVP_EARG; // fixed at creation
+    // The ids are checked and copied in one pass, so what the kernels index
+    // with is exactly what was checked, whatever the caller does with its array.
+    Pending p;
+    p.keys.resize((size_t)r->count);
     std::lock_guard<std::mutex> g(S->mu);
-    if (r->count > S->max_batch) return TMVP_EARG;
     const Store &st = S->store[r->param_set];
     if (r->op == TMVP_SRV_DECAPS && !st.has_sk) return TMVP_EARG;
-    for (int64_t i = 0; i < r->count; i++)
-        if (r->keys[i] < 0 || r->keys[i] >= st.n) return TMVP_EARG;
-    Pending p;
+    for (int64_t i = 0; i < r->count; i++) {
+        const int32_t k = r->keys[i];
+        if (k < 0 || k >= st.n) return TMVP_EARG;
+        p.keys[(size_t)i] = k;
+    }
     p.id = S->next_id++;
     p.req = *r;
+    p.req.keys = nullptr;
     p.t_submit = S->now_us();
     p.pinned_in = pinned_in;
     p.pinned_out = pinned_out;
@@ -613,7 +627,7 @@ int tmvp_srv_submit(tmvp_server *S, const tmvp_srv_request *r, uint64_t *id)
     p.dev_ct = static_cast<uint8_t *>(const_cast<void *>(dev_ct));
     p.dev_ss = static_cast<uint8_t *>(const_cast<void *>(dev_ss));
     p.dev_fail = static_cast<uint8_t *>(const_cast<void *>(dev_fail));
-    S->queue[r->param_set][r->op].push_back(p);
+    S->queue[r->param_set][r->op].push_back(std::move(p));
     S->pending_requests += 1;
     S->stats.requests_submitted += 1;
     if (id) *id = p.id;

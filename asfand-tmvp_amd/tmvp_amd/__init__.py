@@ -34,6 +34,8 @@ MFMA, VALU, SCHOOLBOOK = 0, 1, 2
 ACCUMULATE = 0x1
 IP_RAW, IP_ENC, IP_DEC = 0, 1, 2
 
+# status codes (include/tmvp.h)
+OK, EARG, EHIP, ENOMEM, EUNSUPPORTED, ERANGE, ENODEV = 0, -1, -2, -3, -4, -5, -6
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "out of memory",
           -4: "unsupported by this backend", -5: "secret coefficient outside the int8 MFMA range",
           -6: "no gfx950 device"}

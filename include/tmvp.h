@@ -72,6 +72,8 @@ enum tmvp_param_set {
 enum tmvp_backend {
     TMVP_BACKEND_MFMA = 0,
     TMVP_BACKEND_VALU = 1,
+    /* An alias of TMVP_BACKEND_VALU, not a third kernel: it runs the same
+     * VALU product, and exists so that reference mode 2 maps one to one. */
     TMVP_BACKEND_SCHOOLBOOK = 2
 };
 
@@ -185,7 +187,12 @@ int tmvp_batch_autotune(tmvp_ctx *ctx, const tmvp_work_desc *work, int n, tmvp_s
  * recorded instead of run; the graph replays it with the same buffers.  A
  * KEM call whose workspace would have to grow fails with TMVP_EUNSUPPORTED
  * during capture: reserve it first with tmvp_kem_reserve.  The legacy NULL
- * stream cannot be captured. */
+ * stream cannot be captured.  A graph holds the KEM workspace buffer of the
+ * stream it was captured on until tmvp_graph_destroy: later calls that grow
+ * that workspace, or hand it to another stream, move to a new buffer and
+ * leave the graph's in place.  The graph shares the buffer with eager KEM
+ * calls on the capture stream, so replay it on that stream (or order the
+ * two with events): a replay overlapping such a call races on the scratch. */
 typedef struct tmvp_graph tmvp_graph;
 int tmvp_graph_begin(tmvp_ctx *ctx, tmvp_stream_t stream);
 int tmvp_graph_end(tmvp_ctx *ctx, tmvp_stream_t stream, tmvp_graph **graph);
@@ -283,9 +290,14 @@ typedef struct {
 } tmvp_srv_stats;
 
 int tmvp_srv_create(tmvp_ctx *ctx, const tmvp_srv_config *config, tmvp_server **server);
-/* Appends n keys (sk may be NULL: encapsulation only); ids first_id.. */
+/* Appends n keys (sk may be NULL: encapsulation only); ids first_id..
+ * When h_sk is given, sk i must hold pk i (sk = s | pk | H(pk) | z), else
+ * TMVP_EARG: keyed decapsulation takes A from the store's pk and b, H(pk)
+ * and z from the sk.  Waits for the batches in flight. */
 int tmvp_srv_add_keys(tmvp_server *server, int param_set, const uint8_t *h_pk, const uint8_t *h_sk, int64_t n,
                       int64_t *first_id);
+/* Checks the request's key ids and copies them: the caller may reuse
+ * request->keys as soon as this returns (the in/out buffers must stay valid). */
 int tmvp_srv_submit(tmvp_server *server, const tmvp_srv_request *request, uint64_t *id);
 /* Up to max completions, waiting up to timeout_us for the first; returns the count. */
 int tmvp_srv_poll(tmvp_server *server, tmvp_srv_completion *out, int max, int timeout_us);

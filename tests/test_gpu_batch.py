@@ -211,6 +211,50 @@ def test_graph_replay_matches_eager(ctx):
         j.free()
 
 
+def test_graph_keeps_its_workspace_when_the_stream_grows(ctx):
+    """A graph holding a KEM capture stays correct after the capture stream's
+    workspace grows for a larger eager call, and after more streams than the
+    context keeps workspaces for have evicted it: the graph's buffer is kept
+    alive and not reused."""
+    rng = np.random.default_rng(81)
+    st = T.Stream()
+    kb, big = 200, 3000
+    pk, _ = O.kem_keypair_batch("saber", rng.integers(0, 256, (big, 96), dtype=np.uint8), nthreads=8)
+    rnd = rng.integers(0, 256, (big, 32), dtype=np.uint8)
+    want_ct, want_ss = O.kem_enc_batch("saber", rnd[:kb], pk[:kb], nthreads=8)
+    dpk, drnd = T.DeviceBuffer.from_numpy(pk), T.DeviceBuffer.from_numpy(rnd)
+    dct, dss = T.DeviceBuffer(kb * 1088), T.DeviceBuffer(kb * 32)
+    ect, ess = T.DeviceBuffer(big * 1088), T.DeviceBuffer(big * 32)
+    ctx.kem_reserve("saber", kb, stream=st)
+    g = ctx.capture(st, lambda: ctx.kem_encaps(dct, dss, dpk, drnd, kb, "saber", T.MFMA, stream=st))
+    others = []
+    try:
+        for rep in range(3):
+            if rep == 1:  # grow the capture stream's workspace
+                ctx.kem_encaps(ect, ess, dpk, drnd, big, "saber", T.MFMA, stream=st)
+                st.synchronize()
+            if rep == 2:  # evict it: 17 more streams each get a workspace
+                others = [T.Stream() for _ in range(17)]
+                for o in others:
+                    ctx.kem_encaps(ect, ess, dpk, drnd, 64, "saber", T.MFMA, stream=o)
+                T.device_synchronize()
+            dct.zero()
+            dss.zero()
+            T.device_synchronize()
+            g.launch(st)
+            st.synchronize()
+            ctx.check(st)
+            assert np.array_equal(dct.download(np.uint8, (kb, -1)), want_ct), rep
+            assert np.array_equal(dss.download(np.uint8, (kb, -1)), want_ss), rep
+    finally:
+        g.destroy()
+        for o in others:
+            o.destroy()
+        st.destroy()
+    for b in (dpk, drnd, dct, dss, ect, ess):
+        b.free()
+
+
 def test_capture_refuses_workspace_growth(ctx):
     st = T.Stream()
     kb = 5000

@@ -74,3 +74,24 @@ def test_reference_host_functions_fresh_randomness(lib):
         assert np.array_equal(k, k2)
         outs.append(ct)
     assert not np.array_equal(outs[0], outs[1])
+
+
+def test_randomness_for_another_batch_is_an_error(lib):
+    """Randomness set for a batch other than the configured one is refused (an
+    error status, nothing written), not silently replaced by fresh randomness;
+    it is dropped, so the next encapsulation draws its own."""
+    batch = 4
+    assert lib.tmvp_host_configure(0, batch, T.SABER, 1, T.SABLE2) == 0
+    pk, sk = O.kem_keypair_batch("saber", np.arange(batch * 96, dtype=np.uint32).astype(np.uint8).reshape(batch, 96))
+    rnd = np.ones((batch - 1, 32), np.uint8)
+    assert lib.tmvp_host_set_randomness(_buf(rnd), batch - 1) == 0
+    ct = np.zeros((batch, 1088), np.uint8)
+    k = np.zeros((batch, 32), np.uint8)
+    lib.saber_enc(0, _buf(k), _buf(pk), _buf(ct))
+    assert lib.tmvp_host_status() == T.EARG
+    assert not ct.any() and not k.any()
+    lib.saber_enc(0, _buf(k), _buf(pk), _buf(ct))
+    assert lib.tmvp_host_status() == 0
+    k2 = np.zeros((batch, 32), np.uint8)
+    lib.saber_dec(0, _buf(ct), _buf(pk), _buf(sk), _buf(k2))
+    assert lib.tmvp_host_status() == 0 and np.array_equal(k, k2)

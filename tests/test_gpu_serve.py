@@ -293,3 +293,74 @@ def test_serving_all_parameter_sets(ctx, backend):
             assert np.array_equal(got, e), key
     for ps in sets:
         assert outs[(ps, 1)][1][::4].all() and not outs[(ps, 1)][1][1::4].any(), ps
+
+
+def test_serving_key_ids_copied_at_submit(ctx, keys):
+    """The server checks and copies a request's key ids at submit: overwriting
+    the caller's id array right after submit (with ids that would be out of
+    range) changes nothing, however late the request's batch is launched."""
+    rng = np.random.default_rng(90)
+    srv = T.Server(ctx, max_batch=2048, slots=2)
+    try:
+        srv.add_keys("saber", *keys["saber"])
+        reqs = []
+        for k in range(24):
+            n = int(rng.integers(200, 2048))
+            ids = rng.integers(0, NKEYS, n).astype(np.int32)
+            keep = ids.copy()
+            rnd = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+            _, out = srv.submit(T.SRV_ENCAPS, "saber", ids, rnd, tag=k)
+            ids[:] = -1  # the caller reuses its id buffer at once
+            reqs.append((keep, rnd, out))
+        done = {}
+        while len(done) < len(reqs):
+            for c in srv.poll(64, timeout_us=200000):
+                done[c["tag"]] = c
+        assert all(c["status"] == 0 for c in done.values())
+    finally:
+        srv.close()
+    for k in (0, 11, 23):
+        keep, rnd, (ct, ss) = reqs[k]
+        want_ct, want_ss = T.run_kem_encaps(ctx, keys["saber"][0][keep], rnd, "saber", T.MFMA)
+        assert np.array_equal(ct, want_ct) and np.array_equal(ss, want_ss), k
+
+
+def test_serving_add_keys_rejects_mismatched_sk(ctx, keys):
+    """An sk that does not hold its own pk is refused: keyed decapsulation
+    would take A from one key and b, H(pk), z from the other."""
+    pk, sk = keys["saber"]
+    srv = T.Server(ctx, max_batch=64, slots=1)
+    try:
+        with pytest.raises(T.TmvpError) as e:
+            srv.add_keys("saber", pk[:8], sk[1:9])
+        assert e.value.code == T.EARG
+        assert srv.add_keys("saber", pk[:8], sk[:8]) == 0  # nothing was added by the refused call
+    finally:
+        srv.close()
+
+
+def test_serving_store_growth_keeps_keys(ctx):
+    """Keys added in several calls across the store's growth steps (capacity
+    1024, then doubled, copying the keys already there) serve the same bytes
+    as the plain batched calls, low and high key ids alike."""
+    rng = np.random.default_rng(91)
+    total = 2300
+    pk, sk = T.run_kem_keypair(ctx, rng.integers(0, 256, (total, 96), dtype=np.uint8), "saber", T.MFMA)
+    srv = T.Server(ctx, max_batch=1024, slots=2)
+    try:
+        first = [srv.add_keys("saber", pk[a:b], sk[a:b]) for a, b in ((0, 700), (700, 1400), (1400, total))]
+        assert first == [0, 700, 1400]
+        ids = np.concatenate([rng.integers(0, 1024, 100), rng.integers(1024, total, 300)]).astype(np.int32)
+        rnd = rng.integers(0, 256, (ids.size, 32), dtype=np.uint8)
+        want_ct, want_ss = T.run_kem_encaps(ctx, pk[ids], rnd, "saber", T.MFMA)
+        _, (ct, ss) = srv.submit(T.SRV_ENCAPS, "saber", ids, rnd, tag=0)
+        _, (ss2, fail) = srv.submit(T.SRV_DECAPS, "saber", ids, want_ct, tag=1)
+        done = {}
+        while len(done) < 2:
+            for c in srv.poll(8, timeout_us=200000):
+                done[c["tag"]] = c
+        assert all(c["status"] == 0 for c in done.values())
+    finally:
+        srv.close()
+    assert np.array_equal(ct, want_ct) and np.array_equal(ss, want_ss)
+    assert np.array_equal(ss2, want_ss) and not fail.any()
