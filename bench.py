@@ -10,8 +10,20 @@ beside it on the same buffers (`valu`), and the two outputs are compared bit
 for bit before timing.
 
 Timing uses HIP events recorded on the stream the kernels are launched on.
+The K timed launches run twice: right after the W warm-up launches
+(`ms_per_step_cold`, the device still in its start-up clock transient) and
+after `settle_launches` more back-to-back launches (`ms_per_step`, the settled
+device, which `value` reports).  `mfma_util_of_published` is the executed
+int8 rate (both limbs) against MI355X's published 5.0 POPS; `roofline.measured`
+holds this device's probed ceilings (HBM streaming with this traffic shape,
+both int8 MFMA shapes) with the shape each was best at.
+
+`polymul` is config 2: 65536 raw products, MFMA against VALU, bit for bit.
+`batch1` is config 1's shape on the GPU: one MatrixVectorMul per call.
 `cpu_baseline` times the CPU oracle (a C restatement of the reference's
-Toom-4/schoolbook path) on a bounded sample of the same workload.
+Toom-4 and schoolbook paths) on bounded samples: the headline shape at one
+thread and at the process's host-core share, raw products, and one
+instance at a time (config 1).
 
 `mixed` reports the mixed-parameter workload: 4096 LightSaber, 4096 Saber and
 4096 FireSaber instances, each a MatrixVectorMul rounded to p plus an
@@ -27,9 +39,10 @@ memory) and instances/s.
 `kem` reports the full batched Saber KEM at 16384 instances (key generation,
 encapsulation, decapsulation with all sampling, hashing, codecs and products on
 the GPU; seeds and keys resident in HBM): ops/s per step and p50/p99 of the
-GPU time of each batched call, after a round-trip check of every instance.
+GPU time of each batched call, after a round-trip check of every instance,
+with its Keccak-f[1600] rate against the VALU-only permutation probe.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-kem] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--settle S] [--no-kem] [--no-cpu]
 """
 import argparse
 import json
@@ -81,21 +94,79 @@ def synthetic_inputs(batch, seed=1):
     return A, s
 
 
-def measured_peaks():
-    """HBM streaming rate with this workload's traffic shape and the int8
-    matrix-core rate, measured on this device (libtmvp_peaks.so)."""
+# MI355X published dense peaks (the roofline's `peak`); the probes below are
+# what this device reaches with this workload's shapes.
+MFMA_I8_PUBLISHED_TOPS = 5000.0
+
+
+def peaks_lib():
     import ctypes
     path = os.path.join(ROOT, "asfand-tmvp_amd", "libtmvp_peaks.so")
     if not os.path.exists(path):
         return None
     lib = ctypes.CDLL(path)
+    lib.tmvp_peak_keccak.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    return lib
+
+
+def measured_peaks():
+    """HBM streaming rate with this workload's traffic shape (at 1-4
+    workgroups of four waves per CU) and the int8 matrix-core rate of both
+    gfx950 int8 MFMA shapes (at 1, 2 and 4 waves per SIMD), measured on this
+    device (libtmvp_peaks.so); the best of each is the measured ceiling."""
+    import ctypes
+    lib = peaks_lib()
+    if lib is None:
+        return None
     us, gbs, tops = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
-    if lib.tmvp_peak_stream(ctypes.c_int64(BATCH), 20, ctypes.byref(us), ctypes.byref(gbs)):
+    stream = {}
+    for w in (1, 2, 3, 4):
+        if lib.tmvp_peak_stream_at(ctypes.c_int64(BATCH), 20, w, ctypes.byref(us), ctypes.byref(gbs)):
+            return None
+        stream[f"{w}_wg_per_cu"] = {"us": us.value, "GBs": gbs.value}
+    mfma = {}
+    for shape, name in ((0, "16x16x64"), (1, "32x32x32")):
+        for wps in (1, 2, 4):
+            if lib.tmvp_peak_mfma_i8_shape(5, shape, wps, ctypes.byref(tops)):
+                return None
+            mfma[f"{name}_{wps}w"] = tops.value
+    best_stream = min(stream.items(), key=lambda kv: kv[1]["us"])
+    best_mfma = max(mfma.items(), key=lambda kv: kv[1])
+    return {"hbm_stream_GBs": best_stream[1]["GBs"], "hbm_stream_us": best_stream[1]["us"],
+            "stream_shape": f"per instance 6144 B read + 1536 B written, one wave per instance, B=65536, "
+                            f"best at {best_stream[0]}",
+            "stream_by_shape": stream,
+            "mfma_i8_tops": best_mfma[1], "mfma_i8_shape": best_mfma[0], "mfma_i8_by_shape": mfma}
+
+
+def keccak_peak():
+    """Keccak-f[1600] permutations/s of the KEM's lane-pair round code on the
+    VALU alone (state in registers), best over 1-8 waves per SIMD."""
+    import ctypes
+    lib = peaks_lib()
+    if lib is None:
         return None
-    if lib.tmvp_peak_mfma_i8(5, ctypes.byref(tops)):
-        return None
-    return {"hbm_stream_GBs": gbs.value, "hbm_stream_us": us.value, "mfma_i8_tops": tops.value,
-            "stream_shape": "per instance 6144 B read + 1536 B written, one wave per instance, B=65536"}
+    rate = ctypes.c_double()
+    by = {}
+    for wps in (1, 2, 4, 8):
+        for u in (1, 2):
+            if lib.tmvp_peak_keccak(5, wps, u, ctypes.byref(rate)):
+                return None
+            by[f"{wps}w_u{u}"] = rate.value
+    best = max(by.items(), key=lambda kv: kv[1])
+    return {"perms_per_s": best[1], "shape": best[0], "by_shape": by}
+
+
+def keccak_perms(p):
+    """Keccak-f[1600] calls per instance of each KEM operation, from the
+    sponge lengths (SHAKE-128 rate 168, SHA3-256 136, SHA3-512 72 bytes)."""
+    shake = lambda n: -(-n // 168)
+    h = lambda n: n // 136 + 1
+    g = lambda n: n // 72 + 1
+    araw, sraw = p["l"] * p["l"] * p["eq"] * 32, p["l"] * p["mu"] * 32
+    return {"keypair": shake(32) + shake(araw) + shake(sraw) + h(p["pk_bytes"]),
+            "encaps": h(32) + h(p["pk_bytes"]) + g(64) + shake(araw) + shake(sraw) + h(p["ct_bytes"]) + h(64),
+            "decaps": g(64) + shake(araw) + shake(sraw) + h(p["ct_bytes"]) + h(64)}
 
 
 def time_launches(fn, stream, steps, warmup):
@@ -116,26 +187,65 @@ def time_launches(fn, stream, steps, warmup):
     return ms
 
 
-def cpu_baseline(seconds=12.0):
-    """Time the CPU oracle (Toom-4 over Karatsuba, the faster of the two CPU
-    restatements) on host cores over a bounded sample of the same workload."""
+# The GPU box gives one GPU's process a 16-core share of its host, whatever
+# the machine's core count: the thread pools use at most that many.
+CPU_SHARE = 16
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_rate(fn, batch, seconds):
+    """Instances per second of fn() (one call = `batch` instances) over about
+    `seconds` of wall time, after one untimed call."""
+    fn()
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        fn()
+        done += batch
+    dt = time.perf_counter() - t0
+    return done / dt, done, dt
+
+
+def cpu_baseline():
+    """Time the CPU oracle (the C restatement of the reference's schoolbook
+    and Toom-4 paths) on host cores over bounded samples of the same
+    workloads: Saber L=3 MatrixVectorMul (the headline shape) and raw products
+    (config 2) at one thread and at the whole pool, and one MatrixVectorMul
+    at a time on one thread (config 1, batch 1)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # test infrastructure: used only as the timed CPU baseline
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    ncpu = host_cores()
+    pool = max(1, min(ncpu, CPU_SHARE))
     A, s = synthetic_inputs(4096, seed=2)
-    O.matvec_batch(L, A[:64], s[:64], method=1, nthreads=cores)  # warm-up
-    done, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        O.matvec_batch(L, A, s, method=1, nthreads=cores)
-        done += A.shape[0]
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "MatVec/s", "cores": cores, "kind": "port",
-            "sample": f"{done} Saber L=3 MatrixVectorMul instances (Toom-4, {cores} threads, {dt:.1f} s)"}
+    legs = {}
+    samples = []
+    for method, mname, secs in ((1, "toom4", 6.0), (0, "schoolbook", 3.0)):
+        for threads, tname, n in ((pool, "pool", 4096), (1, "1thread", 256)):
+            sec = secs if threads == pool else secs / 2
+            rate, done, dt = cpu_rate(lambda: O.matvec_batch(L, A[:n], s[:n], method=method, nthreads=threads),
+                                      n, sec)
+            legs[f"matvec_{mname}_{tname}_per_s"] = rate
+            samples.append(f"{done} MatVec {mname} {threads}t {dt:.1f}s")
+    a = A.reshape(-1, N)[:8192]
+    sp = s.reshape(-1, N)[:8192]
+    for threads, tname, n in ((pool, "pool", 8192), (1, "1thread", 1024)):
+        rate, done, dt = cpu_rate(lambda: O.polymul_batch(a[:n], sp[:n], method=1, nthreads=threads), n, 2.0)
+        legs[f"polymul_toom4_{tname}_per_s"] = rate
+        samples.append(f"{done} polymul toom4 {threads}t {dt:.1f}s")
+    for method, mname in ((0, "schoolbook"), (1, "toom4")):
+        O.bench_matvec1(L, A[0], s[0], method, 200)
+        legs[f"batch1_matvec_{mname}_ns"] = O.bench_matvec1(L, A[0], s[0], method, 20000 if method else 10000)
+    return {"value": legs["matvec_toom4_pool_per_s"], "unit": "MatVec/s", "cores": pool, "kind": "port",
+            "host_cores": ncpu,
+            "sample": f"Saber L=3 MatrixVectorMul, Toom-4 over Karatsuba, {pool} threads (the GPU's {CPU_SHARE}-core "
+                      f"host share; the host has {ncpu}); legs: " + "; ".join(samples),
+            "legs": legs}
 
 
 def percentile(xs, q):
@@ -189,9 +299,14 @@ def kem_bench(ctx, stream, batch=16384, reps=100, ps="saber", cpu_seconds=4.0, w
     for e in ev:
         e.destroy()
     ctx.check(stream)
+    perms = keccak_perms(p)
+    kf = batch * (perms["keypair"] + perms["encaps"] + perms["decaps"]) / (
+        (np.mean(t_kp) + np.mean(t_enc) + np.mean(t_dec)) / 1e3)
     out = {
         "workload": f"Saber KEM keypair/encaps/decaps, {batch} instances per call",
         "batch": batch, "reps": reps,
+        "keccak_f_per_instance": perms,
+        "keccak_f_per_s": kf,
         "keypair_per_s": batch / (np.mean(t_kp) / 1e3),
         "encaps_per_s": batch / (np.mean(t_enc) / 1e3),
         "decaps_per_s": batch / (np.mean(t_dec) / 1e3),
@@ -208,7 +323,7 @@ def kem_bench(ctx, stream, batch=16384, reps=100, ps="saber", cpu_seconds=4.0, w
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O  # test infrastructure: used only as the timed CPU baseline
 
-        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+        cores = max(1, min(host_cores(), CPU_SHARE))
         n = 1024
         seeds_h = np.random.default_rng(4).integers(0, 256, (n, 96), dtype=np.uint8)
         rnd_h = np.random.default_rng(5).integers(0, 256, (n, 32), dtype=np.uint8)
@@ -391,6 +506,69 @@ def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, s
     }
 
 
+def polymul_bench(ctx, stream, steps, batch=65536):
+    """Raw batched negacyclic products (config 2): B = 65536 products of a
+    uniform 13-bit a by a CBD(mu=8) s, out mod 2^13, MFMA and VALU forms on
+    the same buffers, outputs compared bit for bit."""
+    rng = np.random.default_rng(12)
+    a = rng.integers(0, 8192, (batch, N), dtype=np.uint16)
+    _, s3 = synthetic_inputs(batch // 3 + 1, seed=13)
+    s = s3.reshape(-1, N)[:batch]
+    da, ds_ = T.DeviceBuffer.from_numpy(a), T.DeviceBuffer.from_numpy(s)
+    om, ov = T.DeviceBuffer(batch * N * 2), T.DeviceBuffer(batch * N * 2)
+    run_m = lambda: ctx.polymul(om, da, ds_, batch, 13, T.MFMA, stream=stream)
+    run_v = lambda: ctx.polymul(ov, da, ds_, batch, 13, T.VALU, stream=stream)
+    run_m(); run_v()
+    stream.synchronize()
+    ctx.check(stream)
+    same = bool(np.array_equal(om.download(np.uint16), ov.download(np.uint16)))
+    if not same:
+        print("[bench] ERROR: polymul MFMA and VALU outputs differ", file=sys.stderr)
+        sys.exit(1)
+    ms_m = time_launches(run_m, stream, steps, 20) / steps
+    ms_v = time_launches(run_v, stream, max(5, steps // 4), 3) / max(5, steps // 4)
+    for b in (da, ds_, om, ov):
+        b.free()
+    algo = 1536 * batch
+    return {"workload": f"{batch} negacyclic products a*s mod (x^256+1, 2^13), a uniform 13-bit, s CBD mu=8",
+            "batch": batch, "mfma_us": ms_m * 1e3, "valu_us": ms_v * 1e3, "outputs_equal": same,
+            "mfma_products_per_s": batch / (ms_m / 1e3), "valu_products_per_s": batch / (ms_v / 1e3),
+            "mfma_over_valu": ms_v / ms_m,
+            "roofline": {"bound": "hbm", "achieved": algo / (ms_m / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": algo / (ms_m / 1e3) / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes": algo,
+                         "floor_us": algo / (HBM_PEAK_GBS * 1e9) * 1e6}}
+
+
+def batch1_bench(ctx, stream, reps=200):
+    """One Saber L=3 MatrixVectorMul with rounding (config 1's shape) per call:
+    the GPU's per-call latency for a single instance (event pairs around each
+    launch, inputs resident), MFMA and VALU."""
+    A, s = synthetic_inputs(1, seed=14)
+    dA, ds_, do = T.DeviceBuffer.from_numpy(A), T.DeviceBuffer.from_numpy(s), T.DeviceBuffer(L * N * 2)
+    out = {"workload": "one Saber L=3 MatrixVectorMul + round q->p per call"}
+    for name, backend in (("mfma", T.MFMA), ("valu", T.VALU)):
+        run = lambda: ctx.matvec(do, dA, ds_, 1, T.SABER, backend, round_to_p=True, stream=stream)
+        for _ in range(20):
+            run()
+        stream.synchronize()
+        e0, e1 = T.Event(), T.Event()
+        ts = []
+        for _ in range(reps):
+            e0.record(stream)
+            run()
+            e1.record(stream)
+            e1.synchronize()
+            ts.append(e0.elapsed_ms(e1) * 1e3)
+        e0.destroy()
+        e1.destroy()
+        out[f"{name}_us_p50"] = percentile(ts, 50)
+        out[f"{name}_us_p99"] = percentile(ts, 99)
+    ctx.check(stream)
+    for b in (dA, ds_, do):
+        b.free()
+    return out
+
+
 def calib_copy(mib):
     """A plain copy of known size: read mib MiB, write mib MiB (runtime copy kernel).
     Its FETCH_SIZE / WRITE_SIZE readings give the scale of those counters."""
@@ -408,9 +586,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     # The device takes about 150 back-to-back launches of the B=65536 kernel
-    # to settle its clocks after start-up (per-launch time 123 -> 166 -> 120 us);
-    # the default warm-up runs past that transient.
-    ap.add_argument("--warmup", type=int, default=300)
+    # to settle its clocks after start-up (per-launch time 123 -> 166 -> 120 us):
+    # the settle phase below runs past that transient and is reported.
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle", type=int, default=400,
+                    help="back-to-back launches between the cold and the settled timing (reported)")
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kem", action="store_true", help="skip the batched KEM leg")
@@ -456,8 +636,16 @@ def main():
         print("[bench] ERROR: MFMA and VALU outputs differ", file=sys.stderr)
         sys.exit(1)
 
-    ms_mfma = time_launches(run_mfma, stream, args.steps, args.warmup)
-    print(f"[bench] mfma {ms_mfma / args.steps * 1e3:.1f} us/launch", file=sys.stderr, flush=True)
+    # Timed twice: right after the W warm-up steps (the device still in its
+    # start-up clock transient) and again after a reported settle phase of
+    # back-to-back launches; `value` is the settled rate, the cold one is
+    # reported beside it.
+    ms_cold = time_launches(run_mfma, stream, args.steps, args.warmup)
+    print(f"[bench] mfma cold {ms_cold / args.steps * 1e3:.1f} us/launch", file=sys.stderr, flush=True)
+    settle_ms = time_launches(run_mfma, stream, args.settle, 0) if args.settle else 0.0
+    ms_mfma = time_launches(run_mfma, stream, args.steps, 0)
+    print(f"[bench] mfma {ms_mfma / args.steps * 1e3:.1f} us/launch after {args.settle} settle launches",
+          file=sys.stderr, flush=True)
     # the same launch captured in a graph: median of 25 replays, each between
     # its own pair of events (so each pays its own launch and ramp-up)
     g = ctx.capture(stream, run_mfma)
@@ -490,7 +678,11 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "settle_launches": args.settle,
+        "settle_ms": settle_ms,
         "ms_per_step": ms_mfma / args.steps,
+        "ms_per_step_cold": ms_cold / args.steps,
+        "value_cold": args.batch / (ms_cold / args.steps / 1e3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -501,19 +693,26 @@ def main():
         "valu": {"value": valu_value, "unit": "MatVec/s", "ms_per_step": ms_valu / valu_steps},
         "mfma_graph_replay_median_us": graph_us,
         "mfma_over_valu": value / valu_value,
-        "polymul_per_s": value * 9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic() if args.batch == BATCH else None},
         "mfma_tops_useful": MACS_PER_INSTANCE * 2 * value / 1e12,
         "outputs_equal": same,
     }
+    executed = MACS_PER_INSTANCE * 2 * 2 * value / 1e12  # two int8 limbs per product
+    result["mfma_executed_tops"] = executed
+    result["mfma_util_of_published"] = executed / MFMA_I8_PUBLISHED_TOPS
     peaks = measured_peaks()
     if peaks:
-        executed = MACS_PER_INSTANCE * 2 * 2 * value / 1e12  # two int8 limbs per product
         result["roofline"]["measured"] = {
             **peaks, "frac_of_stream": achieved / peaks["hbm_stream_GBs"],
-            "mfma_executed_tops": executed, "mfma_util_of_measured": executed / peaks["mfma_i8_tops"]}
+            "mfma_executed_tops": executed, "mfma_util_of_measured": executed / peaks["mfma_i8_tops"],
+            "mfma_util_of_published": executed / MFMA_I8_PUBLISHED_TOPS}
+    print("[bench] raw polymul B=65536 ...", file=sys.stderr, flush=True)
+    result["polymul"] = polymul_bench(ctx, stream, args.steps)
+    print(f"[bench] polymul mfma {result['polymul']['mfma_us']:.1f} us, valu {result['polymul']['valu_us']:.1f} us",
+          file=sys.stderr, flush=True)
+    result["batch1"] = batch1_bench(ctx, stream)
     if not args.no_mixed:
         print("[bench] mixed-parameter batch ...", file=sys.stderr, flush=True)
         result["mixed"] = mixed_bench(ctx, stream)
@@ -523,6 +722,12 @@ def main():
     if not args.no_kem:
         print(f"[bench] KEM B={args.kem_batch} reps={args.kem_reps} ...", file=sys.stderr, flush=True)
         result["kem"] = kem_bench(ctx, stream, args.kem_batch, args.kem_reps, with_cpu=not args.no_cpu)
+        kp = keccak_peak()
+        if kp:
+            result["kem"]["roofline"] = {"bound": "valu (Keccak-f[1600])", "achieved": result["kem"]["keccak_f_per_s"],
+                                         "peak": kp["perms_per_s"], "unit": "Keccak-f/s",
+                                         "frac": result["kem"]["keccak_f_per_s"] / kp["perms_per_s"],
+                                         "peak_shape": kp["shape"], "peak_by_shape": kp["by_shape"]}
         print(f"[bench] KEM encaps+decaps p99 {result['kem']['encaps_decaps_ms']['p99']:.2f} ms", file=sys.stderr,
               flush=True)
     if not args.no_serving:

@@ -66,6 +66,7 @@ def _declare(L):
         "or_polymul_batch": (None, [P, P, P, i64, i, i]),
         "or_matvec_batch": (None, [i, P, P, P, i64, i, i, i]),
         "or_innerprod_batch": (None, [i, P, P, P, i64, i, i]),
+        "or_bench_matvec1": (ctypes.c_double, [i, P, P, i, i]),
         "or_round_q2p": (ctypes.c_uint16, [i, ctypes.c_uint16]),
         "or_round_enc_v": (ctypes.c_uint16, [i, ctypes.c_uint16, ctypes.c_uint16]),
         "or_round_dec_m": (ctypes.c_uint16, [i, ctypes.c_uint16, ctypes.c_uint16]),
@@ -173,6 +174,13 @@ def innerprod_batch(l, b, s, method=0, nthreads=1, out=None):
         out = _u16(out).copy().reshape(-1)
     lib().or_innerprod_batch(l, _p(b), _p(s), _p(out), batch, method, nthreads)
     return out.reshape(batch, N)
+
+
+def bench_matvec1(l, A, s, method=0, reps=1000):
+    """Mean ns of one single-instance MatrixVectorMul timed inside the C
+    oracle (one thread); used only as the CPU baseline's batch-1 timing."""
+    A, s = _u16(A).reshape(-1), _u16(s).reshape(-1)
+    return lib().or_bench_matvec1(l, _p(A), _p(s), method, reps)
 
 
 def round_q2p(ps, x):

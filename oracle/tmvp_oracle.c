@@ -8,11 +8,13 @@
  * Reference citations are relative to the reference root:
  *   S/  = TMVP-Saber/SABER-GPU/      SB/ = TMVP_SABLE/
  */
+#define _POSIX_C_SOURCE 200809L /* clock_gettime */
 #include "tmvp_oracle.h"
 
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 /* ------------------------------------------------------------------------ */
 /* Parameter sets: S/include/params.h:1-50, SB/include/SABLE_params.h:1-73.  */
@@ -328,6 +330,25 @@ void or_innerprod(int l, const uint16_t *b, const uint16_t *s, uint16_t *out, in
         f(b + (size_t)j * OR_N, s + (size_t)j * OR_N, t);
         for (int k = 0; k < OR_N; k++) out[k] = (uint16_t)(out[k] + t[k]);
     }
+}
+
+/* Mean wall time (ns) of one single-instance MatrixVectorMul, out zeroed and
+ * recomputed `reps` times on this thread: the batch-1 CPU timing (no call
+ * overhead from the caller per instance). */
+double or_bench_matvec1(int l, const uint16_t *A, const uint16_t *s, int method, int reps)
+{
+    uint16_t out[4 * OR_N];
+    volatile uint16_t sink = 0;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int r = 0; r < reps; r++) {
+        memset(out, 0, sizeof(out));
+        or_matvec(l, A, s, out, 0, method);
+        sink = (uint16_t)(sink + out[r % (l * OR_N)]);
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    (void)sink;
+    return ((double)(t1.tv_sec - t0.tv_sec) * 1e9 + (double)(t1.tv_nsec - t0.tv_nsec)) / (reps > 0 ? reps : 1);
 }
 
 /* ---- pthread batch drivers ---- */

@@ -64,6 +64,9 @@ void or_polymul_toom4(const uint16_t a[OR_N], const uint16_t s[OR_N], uint16_t c
 void or_matvec(int l, const uint16_t *A, const uint16_t *s, uint16_t *out, int transpose, int method);
 void or_innerprod(int l, const uint16_t *b, const uint16_t *s, uint16_t *out, int method);
 
+/* Mean ns of one single-instance MatrixVectorMul (l <= 4), repeated reps times. */
+double or_bench_matvec1(int l, const uint16_t *A, const uint16_t *s, int method, int reps);
+
 /* Batched drivers over independent instances, nthreads >= 1 (pthreads). */
 void or_polymul_batch(const uint16_t *a, const uint16_t *s, uint16_t *c, int64_t batch,
                       int method, int nthreads);
