@@ -2,6 +2,7 @@
 // backend dispatch.  Every entry point returns a status code; none aborts.
 #include <string.h>
 
+#include <mutex>
 #include <new>
 
 #include "../../include/tmvp.h"
@@ -52,6 +53,7 @@ struct tmvp_ctx {
     int device;
     int cus;
     int *d_status; // device word set by MFMA launches on out-of-range secrets
+    uint8_t *scratch; // this device's copy of `scratch` (tmvp_kem_encaps_noalloc)
     Workspace ws[kMaxWorkspaces];
     int nws;
     uint64_t ws_clock;
@@ -76,6 +78,33 @@ constexpr PS kParams[TMVP_NUM_PARAM_SETS] = {
     {2, 13, 10, 3, 10, 0}, {3, 13, 10, 4, 8, 0}, {4, 13, 10, 6, 6, 0},
     {2, 11, 9, 3, 2, 1},   {3, 11, 9, 5, 2, 1},  {4, 11, 10, 3, 2, 1},
 };
+
+// Scratch of tmvp_kem_encaps_noalloc: a fixed device array instead of the
+// context's workspace, so that call allocates nothing.  Every GPU the library
+// runs on holds its own copy from module load until the process ends.  Host
+// code never passes the name `scratch` to a kernel (on the host it is only a
+// stand-in for the symbol): it uses the device address that
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device by noalloc_scratch().
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch[kMaxDevices]; // device address of `scratch` per device, null until looked up
+
+// This device's address of `scratch` (the device must be current).
+uint8_t *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+        g_scratch[device] = static_cast<uint8_t *>(ptr);
+    }
+    return g_scratch[device];
+}
 
 inline int h1_of(const PS &p) { return 1 << (p.eq - p.ep - 1); }
 inline int h2_of(const PS &p) { return (1 << (p.ep - 2)) - (1 << (p.ep - p.et - 1)) + h1_of(p); }
@@ -456,6 +485,11 @@ int tmvp_init(tmvp_ctx **ctx, int device)
     if (!c) return TMVP_ENOMEM;
     c->device = device;
     c->d_status = nullptr;
+    c->scratch = noalloc_scratch(device); // set-up of tmvp_kem_encaps_noalloc, once per device
+    if (!c->scratch) {
+        delete c;
+        return TMVP_EHIP;
+    }
     c->nws = 0;
     c->ws_clock = 0;
     c->next_plan = 0;
@@ -610,6 +644,28 @@ int tmvp_kem_decaps(tmvp_ctx *ctx, int ps, int backend, uint8_t *d_ss, const uin
     int rc = kem_setup(ctx, ps, batch, stream, P, W);
     if (rc) return rc;
     return hip_status(kem_decaps(P, W, backend == TMVP_BACKEND_MFMA ? 0 : 1, d_ct, d_sk, d_ss, d_fail, batch,
+                                 ctx->d_status, reinterpret_cast<hipStream_t>(stream)));
+}
+
+int tmvp_kem_encaps_noalloc(tmvp_ctx *ctx, int ps, int backend, uint8_t *d_ct, uint8_t *d_ss, const uint8_t *d_pk,
+                            const uint8_t *d_rnd, int64_t batch, tmvp_stream_t stream)
+{
+    if (!kem_args_ok(ctx, ps, backend, batch)) return TMVP_EARG;
+    if (batch == 0) return TMVP_OK;
+    if (!d_ct || !d_ss || !d_pk || !d_rnd) return TMVP_EARG;
+    const KemParams P = kem_params(ps);
+    if (kem_workspace_bytes(P, batch) > kNoAllocScratchBytes) return TMVP_ENOMEM; // the array does not hold it
+    int rc = set_device(ctx);
+    if (rc) return rc;
+    if (!ctx->scratch) return TMVP_EHIP;
+    // The whole chain runs on `stream` (no side stream, no events), and every
+    // scratch word it reads it has written earlier in the same call, so
+    // nothing needs clearing between calls.
+    KemWorkspace W;
+    kem_workspace_carve(P, batch, ctx->scratch, W);
+    W.aux = nullptr;
+    W.fork = W.join = nullptr;
+    return hip_status(kem_encaps(P, W, backend == TMVP_BACKEND_MFMA ? 0 : 1, d_pk, d_rnd, d_ct, d_ss, batch,
                                  ctx->d_status, reinterpret_cast<hipStream_t>(stream)));
 }
 

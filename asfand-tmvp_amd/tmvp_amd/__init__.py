@@ -70,7 +70,7 @@ EXPORTS = ("tmvp_init", "tmvp_destroy", "tmvp_ctx_status", "tmvp_strerror", "tmv
            "tmvp_host_configure", "tmvp_host_set_randomness", "tmvp_host_status", "tmvp_host_release",
            "saber_enc", "saber_dec", "SABLE_enc", "SABLE_dec",
            "tmvp_batch_launch", "tmvp_batch_autotune", "tmvp_graph_begin", "tmvp_graph_end", "tmvp_graph_launch",
-           "tmvp_graph_destroy", "tmvp_kem_reserve",
+           "tmvp_graph_destroy", "tmvp_kem_reserve", "tmvp_kem_encaps_noalloc",
            "tmvp_srv_create", "tmvp_srv_add_keys", "tmvp_srv_submit", "tmvp_srv_poll", "tmvp_srv_drain",
            "tmvp_srv_get_stats", "tmvp_srv_destroy")
 
@@ -147,6 +147,7 @@ def load_library(path=LIB_PATH):
         "tmvp_graph_launch": (i, [P, P]),
         "tmvp_graph_destroy": (i, [P]),
         "tmvp_kem_reserve": (i, [P, i, i64, P]),
+        "tmvp_kem_encaps_noalloc": (i, [P, i, i, P, P, P, P, i64, P]),
         "tmvp_srv_create": (i, [P, ctypes.POINTER(SrvConfig), ctypes.POINTER(P)]),
         "tmvp_srv_add_keys": (i, [P, i, P, P, i64, ctypes.POINTER(i64)]),
         "tmvp_srv_submit": (i, [P, ctypes.POINTER(SrvRequest), ctypes.POINTER(ctypes.c_uint64)]),
@@ -155,6 +156,12 @@ def load_library(path=LIB_PATH):
         "tmvp_srv_get_stats": (i, [P, ctypes.POINTER(SrvStats)]),
         "tmvp_srv_destroy": (i, [P]),
     }
+    # Every export needs its signature: without one ctypes passes 64-bit
+    # arguments as 32-bit ints (pointers truncated, int64 sizes with garbage
+    # upper halves when they go on the stack).
+    missing = set(EXPORTS) - set(sigs)
+    if missing:
+        raise RuntimeError(f"no ctypes signature for {sorted(missing)}")
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
         f.restype, f.argtypes = res, args

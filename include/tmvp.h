@@ -230,6 +230,21 @@ int tmvp_kem_encaps(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_ct, ui
 int tmvp_kem_decaps(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_ss, const uint8_t *d_ct,
                     const uint8_t *d_sk, uint8_t *d_fail, int64_t batch, tmvp_stream_t stream);
 
+/* Encapsulation that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as tmvp_kem_encaps, but its scratch is a fixed
+ * 32 MiB device array of the library (one per GPU, held from load until the
+ * process ends) instead of the context's workspace: the call makes no
+ * hipMalloc / hipFree, creates no stream or event, and runs on `stream` only.
+ * A batch must fit the array: at most 5318 instances for L = 2 (LightSaber,
+ * Sable-1), 3224 for L = 3 (Saber, Sable-2) and 2161 for L = 4 (FireSaber,
+ * Sable-3); a larger batch returns TMVP_ENOMEM without running anything.
+ * Every call on one GPU uses that GPU's one array: calls that overlap on a
+ * GPU (from any context, thread or stream) race on it. */
+int tmvp_kem_encaps_noalloc(tmvp_ctx *ctx, int param_set, int backend, uint8_t *d_ct, uint8_t *d_ss,
+                            const uint8_t *d_pk, const uint8_t *d_rnd, int64_t batch, tmvp_stream_t stream);
+
 /* Grow the KEM workspace of `stream` to `batch` instances of `param_set` now
  * (synchronises the device), so later KEM calls on it allocate nothing. */
 int tmvp_kem_reserve(tmvp_ctx *ctx, int param_set, int64_t batch, tmvp_stream_t stream);
