@@ -39,8 +39,11 @@ struct KeyStore {
     const uint8_t *hpk;
 };
 
-size_t kem_workspace_bytes(const KemParams &P, int64_t B);
-void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorkspace &W);
+// Bytes of a workspace for B instances; with_A = false leaves out the matrix
+// (W.A = null), which only keyed calls on the MFMA backend may use: they read
+// A from the key store.
+size_t kem_workspace_bytes(const KemParams &P, int64_t B, bool with_A = true);
+void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorkspace &W, bool with_A = true);
 
 // backend: 0 = MFMA, otherwise VALU.  seeds: [B][96] = seed_A | seed_s | z.
 hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, const uint8_t *seeds, uint8_t *pk,

@@ -1195,11 +1195,12 @@ hipError_t kem_store_derive(const KemParams &P, const uint8_t *pk, int64_t n, ui
     return hipGetLastError();
 }
 
-size_t kem_workspace_bytes(const KemParams &P, int64_t B)
+// The matrix A is the last piece, so a workspace without it (keyed MFMA
+// calls read A from the key store) is the same layout cut short.
+size_t kem_workspace_bytes(const KemParams &P, int64_t B, bool with_A)
 {
     const size_t L = (size_t)P.l, n = (size_t)B;
     size_t b = 0;
-    b += n * L * L * kN * 2; // A
     b += n * L * kN * 2;     // s
     b += n * L * kN * 2;     // v1
     b += n * L * kN * 2;     // v2
@@ -1209,10 +1210,12 @@ size_t kem_workspace_bytes(const KemParams &P, int64_t B)
     b += n * 64;             // kr
     b += n * 4;              // diff
     b += n * 32;             // hct
-    return b + 10 * 256;     // each of the 10 pieces (kem_workspace_carve) rounded up to 256 bytes
+    b += 9 * 256;            // each of the pieces above rounded up to 256 bytes (kem_workspace_carve)
+    if (with_A) b += n * L * L * kN * 2 + 256;
+    return b;
 }
 
-void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorkspace &W)
+void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorkspace &W, bool with_A)
 {
     const size_t L = (size_t)P.l, n = (size_t)B;
     auto take = [&](size_t bytes) {
@@ -1220,7 +1223,6 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
         base += (bytes + 255) & ~(size_t)255;
         return p;
     };
-    W.A = reinterpret_cast<uint16_t *>(take(n * L * L * kN * 2));
     W.s = reinterpret_cast<uint16_t *>(take(n * L * kN * 2));
     W.v1 = reinterpret_cast<uint16_t *>(take(n * L * kN * 2));
     W.v2 = reinterpret_cast<uint16_t *>(take(n * L * kN * 2));
@@ -1230,6 +1232,7 @@ void kem_workspace_carve(const KemParams &P, int64_t B, uint8_t *base, KemWorksp
     W.kr = take(n * 64);
     W.diff = reinterpret_cast<uint32_t *>(take(n * 4));
     W.hct = take(n * 32);
+    W.A = with_A ? reinterpret_cast<uint16_t *>(take(n * L * L * kN * 2)) : nullptr;
 }
 
 } // namespace tmvp

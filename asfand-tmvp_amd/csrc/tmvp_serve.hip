@@ -7,16 +7,20 @@
 // Design:
 //  - Key store per parameter set, in HBM: pk, sk, and the matrix A and H(pk)
 //    derived once per key, so a request never runs the GenMatrix sponge.
-//  - Slots: each slot is a stream with its own workspace, device and pinned
-//    host staging, sized for max_batch instances of the largest parameter
-//    set.  The HBM budget fixes how many slots exist; a batch is admitted when
-//    a slot is free, so batches of different parameter sets and operations
-//    run concurrently on the device.
+//  - HBM budget: the key stores and the batches in flight share it.  Each
+//    batch reserves exactly the device bytes its (parameter set, operation,
+//    instances) needs - workspace, key ids, inputs, outputs - from one arena
+//    allocated at creation, and gives them back when it retires.  A batch is
+//    admitted when a stream is free and its bytes fit the budget left: small
+//    batches run side by side where one large batch would take it all, and
+//    a request that does not fit yet waits in its queue.
+//  - Slots: each slot is a stream with its own pinned host staging; their
+//    number bounds the batches in flight.
 //  - A scheduler thread forms each batch from the queue (parameter set,
 //    operation) whose oldest request waited longest, taking requests in
-//    arrival order up to max_batch instances, and retires finished slots by
-//    polling their events.  Requests arriving meanwhile join the next batch
-//    that a free slot takes.
+//    arrival order up to max_batch instances and up to the budget left, and
+//    retires finished slots by polling their events.  Requests arriving
+//    meanwhile join the next batch that a free slot takes.
 //  - Data crosses the bus through a copy kernel that reads or writes
 //    page-locked host memory directly, once per batch each way: the key ids
 //    and pageable inputs from the slot's pinned staging, pinned inputs from
@@ -27,6 +31,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -55,6 +60,7 @@ struct Store {
     uint16_t *A = nullptr;
     bool has_sk = true;
     int64_t n = 0, cap = 0;
+    int64_t bytes = 0; // device bytes held (cap keys)
 };
 
 struct Pending {
@@ -123,9 +129,10 @@ __global__ __launch_bounds__(256) void copy_kernel(const CopyDesc *descs, int n)
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    uint8_t *ws = nullptr;    // KEM workspace
-    uint8_t *d_in = nullptr;  // keys | inputs
-    uint8_t *d_out = nullptr; // ct | ss | fail
+    // device memory of the batch in flight, a range of the server's arena:
+    // workspace | keys | inputs | outputs (ct | ss | fail)
+    size_t off = 0, bytes = 0;
+    uint8_t *ws = nullptr, *d_in = nullptr, *d_out = nullptr;
     int *d_status = nullptr;
     uint8_t *h_in = nullptr, *h_out = nullptr; // pinned staging
     const uint8_t *h_in_dev = nullptr;         // the device's addresses of h_in,
@@ -147,8 +154,13 @@ struct tmvp_server {
     tmvp_ctx *ctx;
     int device, backend;
     int64_t max_batch;
-    size_t ws_bytes, in_bytes, out_bytes;
+    size_t in_bytes, out_bytes; // pinned staging per slot (max_batch of the largest set)
     int64_t desc_cap; // copy-in pieces one batch can need
+    // HBM: the budget covers the key stores and the batches in flight
+    int64_t budget = 0, store_bytes = 0, inflight_bytes = 0;
+    int64_t max_batch_bytes = 0; // one batch of max_batch instances, largest set and operation
+    uint8_t *arena = nullptr;
+    size_t arena_bytes = 0;
     Clock::time_point t0;
     Store store[TMVP_NUM_PARAM_SETS];
     std::vector<Slot> slots;
@@ -171,19 +183,56 @@ namespace {
 constexpr size_t kAlign = 256;
 size_t up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
-// Largest per-instance sizes over all parameter sets.
-void max_sizes(int64_t B, size_t &ws, size_t &in, size_t &out)
+// Keyed MFMA calls read A from the key store; the VALU backend gathers a copy.
+bool ws_with_A(int backend) { return backend != TMVP_BACKEND_MFMA; }
+
+// Device bytes of one batch: workspace | keys | inputs | outputs.
+struct BatchLayout {
+    size_t ws, keys, in, out, total;
+};
+BatchLayout batch_layout(const KemParams &P, int op, int64_t B, int backend)
 {
-    ws = in = out = 0;
+    BatchLayout b;
+    const size_t n = (size_t)B;
+    b.ws = up(kem_workspace_bytes(P, B, ws_with_A(backend)));
+    b.keys = up(n * 4);
+    b.in = up(n * (op == TMVP_SRV_ENCAPS ? 32 : (size_t)P.ct));
+    b.out = up(op == TMVP_SRV_ENCAPS ? n * (P.ct + 32) : n * 33);
+    b.total = b.ws + b.keys + b.in + b.out;
+    return b;
+}
+
+// Pinned staging sizes for max_batch instances of the largest parameter set,
+// and the device bytes of the largest max_batch batch.
+void max_sizes(int64_t B, int backend, size_t &in, size_t &out, size_t &batch)
+{
+    in = out = batch = 0;
     for (int ps = 0; ps < TMVP_NUM_PARAM_SETS; ps++) {
         const KemParams P = kem_params_of(ps);
-        const size_t w = kem_workspace_bytes(P, B);
         const size_t i = up((size_t)B * 4) + up((size_t)B * (size_t)(P.ct > 32 ? P.ct : 32));
         const size_t o = (size_t)B * (P.ct + 32 + 1);
-        ws = w > ws ? w : ws;
         in = i > in ? i : in;
         out = o > out ? o : out;
+        for (int op = 0; op < 2; op++) {
+            const size_t t = batch_layout(P, op, B, backend).total;
+            batch = t > batch ? t : batch;
+        }
     }
+}
+
+// Arena bytes with key stores of `store` bytes: the budget left, but never
+// more than `slots` batches of max_batch instances can use at once.
+int64_t arena_size(const tmvp_server *S, int64_t store)
+{
+    const int64_t most = S->max_batch_bytes * (int64_t)S->slots.size();
+    const int64_t left = S->budget - store;
+    return left < most ? left : most;
+}
+
+// Device bytes of a key store of `cap` keys.
+int64_t store_bytes_of(const KemParams &P, int64_t cap, bool with_sk)
+{
+    return cap * ((int64_t)P.pk + 32 + (int64_t)P.l * P.l * kN * 2 + (with_sk ? (int64_t)P.sk : 0));
 }
 
 void free_slot(Slot &s)
@@ -191,9 +240,6 @@ void free_slot(Slot &s)
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.stream) (void)hipStreamDestroy(s.stream);
-    if (s.ws) (void)hipFree(s.ws);
-    if (s.d_in) (void)hipFree(s.d_in);
-    if (s.d_out) (void)hipFree(s.d_out);
     if (s.d_status) (void)hipFree(s.d_status);
     if (s.h_in) (void)hipHostFree(s.h_in);
     if (s.h_out) (void)hipHostFree(s.h_out);
@@ -206,9 +252,7 @@ int alloc_slot(tmvp_server *S, Slot &s)
 {
     if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return TMVP_EHIP;
     if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) return TMVP_EHIP;
-    if (hipMalloc(&s.ws, S->ws_bytes) != hipSuccess || hipMalloc(&s.d_in, S->in_bytes) != hipSuccess ||
-        hipMalloc(&s.d_out, S->out_bytes) != hipSuccess || hipMalloc(&s.d_status, sizeof(int)) != hipSuccess)
-        return TMVP_ENOMEM;
+    if (hipMalloc(&s.d_status, sizeof(int)) != hipSuccess) return TMVP_ENOMEM;
     if (hipHostMalloc(&s.h_in, S->in_bytes, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&s.h_out, S->out_bytes, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&s.h_status, sizeof(int), hipHostMallocDefault) != hipSuccess ||
@@ -269,7 +313,7 @@ int launch_slot(tmvp_server *S, Slot &s)
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)nd), dim3(256), 0, s.stream, s.h_desc_dev, nd);
     if (hipGetLastError() != hipSuccess) return TMVP_EHIP;
     KemWorkspace W;
-    kem_workspace_carve(P, B, s.ws, W);
+    kem_workspace_carve(P, B, s.ws, W, ws_with_A(S->backend));
     // keyed calls run on the slot's one stream (A is resident, so the
     // decapsulation has no side-stream work)
     W.aux = nullptr;
@@ -364,11 +408,44 @@ void publish(tmvp_server *S, Slot &s, std::vector<tmvp_srv_completion> &cs)
     S->stats.instances_done += s.instances;
     S->stats.batches += 1;
     S->inflight -= 1;
+    S->inflight_bytes -= (int64_t)s.bytes;
+    s.bytes = 0;
     s.busy = false;
     s.reqs.clear();
 }
 
-// Form the next batch: the (set, op) queue whose head waited longest (lock held).
+// Free arena bytes for one more batch: the budget left after the key stores
+// and the batches in flight, and the largest gap between those batches'
+// ranges of the arena (lock held).  *at receives the gap's offset.
+size_t room(const tmvp_server *S, size_t *at)
+{
+    std::vector<std::pair<size_t, size_t>> used;
+    for (const Slot &o : S->slots)
+        if (o.busy && o.bytes) used.emplace_back(o.off, o.bytes);
+    std::sort(used.begin(), used.end());
+    size_t best = 0, best_at = 0, cur = 0;
+    for (const auto &u : used) {
+        if (u.first - cur > best) {
+            best = u.first - cur;
+            best_at = cur;
+        }
+        cur = u.first + u.second;
+    }
+    if (S->arena_bytes - cur > best) {
+        best = S->arena_bytes - cur;
+        best_at = cur;
+    }
+    const int64_t left = S->budget - S->store_bytes - S->inflight_bytes;
+    if (left <= 0) return 0;
+    *at = best_at;
+    return best < (size_t)left ? best : (size_t)left;
+}
+
+// Form the next batch (lock held): from the (set, op) queue whose head waited
+// longest, requests in arrival order while the batch stays within max_batch
+// instances and its device bytes within the room left.  Returns false, and
+// leaves every queue as it was, when not even that queue's first request fits
+// now: it keeps its place, and nothing behind it overtakes it.
 bool pick_batch(tmvp_server *S, Slot &s)
 {
     int bps = -1, bop = -1;
@@ -384,16 +461,34 @@ bool pick_batch(tmvp_server *S, Slot &s)
         }
     if (bps < 0) return false;
     auto &q = S->queue[bps][bop];
+    const KemParams P = kem_params_of(bps);
+    size_t at = 0;
+    const size_t avail = room(S, &at);
+    int64_t n = 0;
+    size_t k = 0;
+    while (k < q.size() && n + q[k].req.count <= S->max_batch &&
+           batch_layout(P, bop, n + q[k].req.count, S->backend).total <= avail) {
+        n += q[k].req.count;
+        k++;
+    }
+    if (k == 0) return false;
     s.ps = bps;
     s.op = bop;
-    s.instances = 0;
+    s.instances = n;
     s.reqs.clear();
-    while (!q.empty() && s.instances + q.front().req.count <= S->max_batch) {
-        s.instances += q.front().req.count;
+    for (size_t i = 0; i < k; i++) {
         s.reqs.push_back(std::move(q.front()));
         q.pop_front();
     }
-    S->pending_requests -= (int64_t)s.reqs.size();
+    const BatchLayout b = batch_layout(P, bop, n, S->backend);
+    s.off = at;
+    s.bytes = b.total;
+    s.ws = S->arena + at;
+    s.d_in = s.ws + b.ws;                 // keys | inputs
+    s.d_out = s.d_in + b.keys + b.in;
+    S->inflight_bytes += (int64_t)b.total;
+    if (S->inflight_bytes > S->stats.max_inflight_bytes) S->stats.max_inflight_bytes = S->inflight_bytes;
+    S->pending_requests -= (int64_t)k;
     return true;
 }
 
@@ -420,7 +515,7 @@ void worker_main(tmvp_server *S)
         }
         for (Slot &s : S->slots) {
             if (s.busy || S->pending_requests == 0) continue;
-            if (!pick_batch(S, s)) break;
+            if (!pick_batch(S, s)) break; // waits for budget: batches in flight give it back
             s.busy = true;
             s.t_start = S->now_us();
             S->inflight += 1;
@@ -466,11 +561,18 @@ void drain_locked(tmvp_server *S, std::unique_lock<std::mutex> &lk)
     S->completed.wait(lk, [S] { return S->inflight == 0 && S->pending_requests == 0; });
 }
 
+// New capacity for `need` keys: exactly `need` for a store's first keys,
+// else at least double the old capacity (fewer copies as keys trickle in).
+int64_t grown_cap(const Store &st, int64_t need)
+{
+    if (need <= st.cap) return st.cap;
+    return st.cap == 0 ? need : (need > 2 * st.cap ? need : 2 * st.cap);
+}
+
 int grow_store(Store &st, const KemParams &P, int64_t need, bool with_sk)
 {
     if (need <= st.cap) return TMVP_OK;
-    int64_t cap = st.cap ? st.cap : 1024;
-    while (cap < need) cap *= 2;
+    const int64_t cap = grown_cap(st, need);
     const size_t abytes = (size_t)P.l * P.l * kN * 2;
     uint8_t *pk = nullptr, *sk = nullptr, *hpk = nullptr;
     uint16_t *A = nullptr;
@@ -497,6 +599,7 @@ int grow_store(Store &st, const KemParams &P, int64_t need, bool with_sk)
     st.hpk = hpk;
     st.A = A;
     st.cap = cap;
+    st.bytes = store_bytes_of(P, cap, with_sk);
     return TMVP_OK;
 }
 
@@ -526,33 +629,43 @@ int tmvp_srv_create(tmvp_ctx *ctx, const tmvp_srv_config *cfg, tmvp_server **out
     S->backend = c.backend;
     S->max_batch = c.max_batch;
     S->t0 = Clock::now();
-    max_sizes(c.max_batch, S->ws_bytes, S->in_bytes, S->out_bytes);
+    size_t batch_max = 0;
+    max_sizes(c.max_batch, c.backend, S->in_bytes, S->out_bytes, batch_max);
+    S->max_batch_bytes = (int64_t)batch_max;
+    S->budget = c.hbm_budget;
     // every request has at least one instance: inputs take at most one piece
     // per request plus in_bytes / kPiece and the keys' pieces, outputs three
     // per request plus out_bytes / kPiece, the staging and the flag
     S->desc_cap = 5 * c.max_batch + (int64_t)((S->in_bytes + S->out_bytes) / kPiece) + 8;
-    const int64_t per_slot = (int64_t)(S->ws_bytes + S->in_bytes + S->out_bytes);
-    int nslots = (int)(c.hbm_budget / per_slot);
-    if (nslots > c.slots) nslots = c.slots;
-    if (nslots < 1) {
+    if (c.hbm_budget < S->max_batch_bytes) {
         delete S;
         return TMVP_ENOMEM; // the budget does not cover one batch of max_batch instances
     }
+    const int nslots = c.slots;
     S->slots.resize(nslots);
+    S->arena_bytes = (size_t)arena_size(S, 0);
+    if (hipMalloc(&S->arena, S->arena_bytes) != hipSuccess) {
+        delete S;
+        return TMVP_ENOMEM;
+    }
     for (Slot &s : S->slots) {
         const int rc = alloc_slot(S, s);
         if (rc) {
             for (Slot &f : S->slots) free_slot(f);
+            (void)hipFree(S->arena);
             delete S;
             return rc;
         }
     }
     S->stats.slots = nslots;
-    S->stats.slot_bytes = per_slot;
+    S->stats.slot_bytes = S->max_batch_bytes;
+    S->stats.hbm_budget = S->budget;
+    S->stats.arena_bytes = (int64_t)S->arena_bytes;
     try {
         S->worker = std::thread(worker_main, S);
     } catch (...) {
         for (Slot &f : S->slots) free_slot(f);
+        (void)hipFree(S->arena);
         delete S;
         return TMVP_ENOMEM;
     }
@@ -577,9 +690,28 @@ int tmvp_srv_add_keys(tmvp_server *S, int ps, const uint8_t *h_pk, const uint8_t
         for (int64_t i = 0; i < n; i++)
             if (memcmp(h_sk + (size_t)i * P.sk + P.indcpa_sk, h_pk + (size_t)i * P.pk, (size_t)P.pk) != 0)
                 return TMVP_EARG;
-    if (st.n == 0) st.has_sk = h_sk != nullptr;
+    const bool with_sk = st.n == 0 ? h_sk != nullptr : st.has_sk;
+    // The stores count against the HBM budget, which must still hold one
+    // batch of max_batch instances after them.
+    const int64_t grown = store_bytes_of(P, grown_cap(st, st.n + n), with_sk);
+    if (S->store_bytes - st.bytes + grown + S->max_batch_bytes > S->budget) return TMVP_ENOMEM;
+    // No batch is in flight: shrink the arena first, so that the arena and
+    // the stores together never hold more device memory than the budget.
+    const int64_t new_arena = arena_size(S, S->store_bytes - st.bytes + grown);
+    if ((size_t)new_arena < S->arena_bytes) {
+        (void)hipFree(S->arena);
+        S->arena = nullptr;
+        S->arena_bytes = 0;
+        if (hipMalloc(&S->arena, (size_t)new_arena) != hipSuccess) return TMVP_ENOMEM;
+        S->arena_bytes = (size_t)new_arena;
+        S->stats.arena_bytes = new_arena;
+    }
+    st.has_sk = with_sk;
+    const int64_t before = st.bytes;
     int rc = grow_store(st, P, st.n + n, st.has_sk);
     if (rc) return rc;
+    S->store_bytes += st.bytes - before;
+    S->stats.store_bytes = S->store_bytes;
     const size_t abytes = (size_t)P.l * P.l * kN * 2;
     rc = TMVP_OK;
     if (hipMemcpy(st.pk + (size_t)st.n * P.pk, h_pk, (size_t)n * P.pk, hipMemcpyHostToDevice) != hipSuccess ||
@@ -676,6 +808,7 @@ int tmvp_srv_destroy(tmvp_server *S)
     }
     S->worker.join();
     for (Slot &s : S->slots) free_slot(s);
+    (void)hipFree(S->arena);
     for (Store &st : S->store) {
         (void)hipFree(st.pk);
         (void)hipFree(st.sk);

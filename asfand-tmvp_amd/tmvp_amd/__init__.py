@@ -97,7 +97,9 @@ class SrvCompletion(ctypes.Structure):
 class SrvStats(ctypes.Structure):
     _fields_ = [("requests_submitted", ctypes.c_int64), ("requests_done", ctypes.c_int64),
                 ("instances_done", ctypes.c_int64), ("batches", ctypes.c_int64), ("slots", ctypes.c_int64),
-                ("slot_bytes", ctypes.c_int64), ("max_concurrent", ctypes.c_int64)]
+                ("slot_bytes", ctypes.c_int64), ("max_concurrent", ctypes.c_int64), ("hbm_budget", ctypes.c_int64),
+                ("store_bytes", ctypes.c_int64), ("arena_bytes", ctypes.c_int64),
+                ("max_inflight_bytes", ctypes.c_int64)]
 
 OP_POLYMUL, OP_MATVEC, OP_MATVEC_T, OP_INNERPROD = 0, 1, 2, 3
 MAX_WORK = 16

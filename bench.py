@@ -501,8 +501,11 @@ def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, s
         "queue_ms": {"p50": percentile(queue, 50), "p99": percentile(queue, 99)},
         "service_ms": {"p50": percentile(service, 50), "p99": percentile(service, 99)},
         "instances_per_s": inst / span_s, "requests_per_s": n_req / span_s,
+        "saturated": n_req / span_s < 0.95 * qps,
         "by_op": by_op, "batches": st["batches"], "mean_batch_instances": st["instances_done"] / max(1, st["batches"]),
         "slots": st["slots"], "max_concurrent_batches": st["max_concurrent"], "setup_s": setup_s,
+        "hbm_budget": st["hbm_budget"], "store_bytes": st["store_bytes"], "arena_bytes": st["arena_bytes"],
+        "max_inflight_bytes": st["max_inflight_bytes"],
     }
 
 
@@ -599,8 +602,10 @@ def main():
     ap.add_argument("--kem-reps", type=int, default=100)
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-parameter leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
-    ap.add_argument("--serve-qps", default="2000,6000", help="offered request rates, comma separated")
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-qps", default="2000,6000,12000,24000,48000",
+                    help="offered request rates, comma separated (a sweep up to past saturation)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="seconds of arrivals per rate, capped at 36000 requests per rate")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -734,7 +739,7 @@ def main():
         result["serving"] = []
         for qps in (float(x) for x in str(args.serve_qps).split(",")):
             print(f"[bench] serving {qps:.0f} req/s ...", file=sys.stderr, flush=True)
-            sv = serving_bench(ctx, qps, args.serve_seconds)
+            sv = serving_bench(ctx, qps, min(args.serve_seconds, 36000.0 / qps))
             result["serving"].append(sv)
             print(f"[bench] serving p50 {sv['p50_ms']:.2f} ms p99 {sv['p99_ms']:.2f} ms, "
                   f"{sv['instances_per_s'] / 1e6:.2f} M instances/s", file=sys.stderr, flush=True)

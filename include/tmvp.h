@@ -255,10 +255,14 @@ int tmvp_kem_reserve(tmvp_ctx *ctx, int param_set, int64_t batch, tmvp_stream_t 
  * once per key.  Requests (one operation, one parameter set, 1..max_batch
  * instances, each under its own key id) are queued per (set, operation); a
  * scheduler thread forms each batch from the queue whose oldest request has
- * waited longest, in arrival order up to max_batch instances, and runs it on
- * a free slot (a stream with its own workspace and staging).  The HBM budget
- * sets the number of slots, so admission waits for budget; batches on
- * different slots overlap on the device.  Host buffers of a request must stay
+ * waited longest, in arrival order up to max_batch instances and up to the
+ * HBM budget left, and runs it on a free slot (a stream with its own pinned
+ * staging).  Each batch reserves exactly the device bytes its parameter set,
+ * operation and size need (workspace, key ids, inputs, outputs), and the key
+ * stores count against the same budget: small batches run side by side where
+ * one large batch would take the budget, and a request that does not fit yet
+ * waits (nothing behind it overtakes it).  Batches on different slots
+ * overlap on the device.  Host buffers of a request must stay
  * valid until its completion is polled; buffers in page-locked memory from
  * hipHostMalloc are read and written by the device in place; other buffers
  * go through the server's pinned staging, copied by the scheduler thread.  Results are those of
@@ -270,9 +274,12 @@ int tmvp_kem_reserve(tmvp_ctx *ctx, int param_set, int64_t batch, tmvp_stream_t 
 typedef struct tmvp_server tmvp_server;
 
 typedef struct {
-    int64_t hbm_budget; /* bytes for in-flight batches; 0 = 4 GiB */
+    int64_t hbm_budget; /* device bytes for the key stores and the batches in
+                           flight together; 0 = 4 GiB.  Must hold one batch of
+                           max_batch instances of the largest set, else
+                           tmvp_srv_create returns TMVP_ENOMEM. */
     int64_t max_batch;  /* instances per batch; 0 = 16384 */
-    int slots;          /* concurrent batches at most; 0 = 4 */
+    int slots;          /* concurrent batches (streams) at most; 0 = 4 */
     int backend;        /* tmvp_backend */
 } tmvp_srv_config;
 
@@ -301,14 +308,22 @@ typedef struct {
 
 typedef struct {
     int64_t requests_submitted, requests_done, instances_done, batches;
-    int64_t slots, slot_bytes, max_concurrent;
+    int64_t slots;              /* streams: batches in flight at most */
+    int64_t slot_bytes;         /* device bytes of one max_batch batch of the largest set */
+    int64_t max_concurrent;     /* most batches in flight at once so far */
+    int64_t hbm_budget;         /* the budget the key stores and batches share */
+    int64_t store_bytes;        /* device bytes the key stores hold */
+    int64_t arena_bytes;        /* device bytes reserved for batches */
+    int64_t max_inflight_bytes; /* most batch bytes in flight at once so far */
 } tmvp_srv_stats;
 
 int tmvp_srv_create(tmvp_ctx *ctx, const tmvp_srv_config *config, tmvp_server **server);
 /* Appends n keys (sk may be NULL: encapsulation only); ids first_id..
  * When h_sk is given, sk i must hold pk i (sk = s | pk | H(pk) | z), else
  * TMVP_EARG: keyed decapsulation takes A from the store's pk and b, H(pk)
- * and z from the sk.  Waits for the batches in flight. */
+ * and z from the sk.  TMVP_ENOMEM (nothing added) when the grown store
+ * would leave the HBM budget less than one max_batch batch.  Waits for the
+ * batches in flight. */
 int tmvp_srv_add_keys(tmvp_server *server, int param_set, const uint8_t *h_pk, const uint8_t *h_sk, int64_t n,
                       int64_t *first_id);
 /* Checks the request's key ids and copies them: the caller may reuse

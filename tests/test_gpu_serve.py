@@ -340,9 +340,9 @@ def test_serving_add_keys_rejects_mismatched_sk(ctx, keys):
 
 
 def test_serving_store_growth_keeps_keys(ctx):
-    """Keys added in several calls across the store's growth steps (capacity
-    1024, then doubled, copying the keys already there) serve the same bytes
-    as the plain batched calls, low and high key ids alike."""
+    """Keys added in several calls, each growing the store (the keys already
+    there are copied to the new allocation), serve the same bytes as the plain
+    batched calls, low and high key ids alike."""
     rng = np.random.default_rng(91)
     total = 2300
     pk, sk = T.run_kem_keypair(ctx, rng.integers(0, 256, (total, 96), dtype=np.uint8), "saber", T.MFMA)
@@ -364,3 +364,61 @@ def test_serving_store_growth_keeps_keys(ctx):
         srv.close()
     assert np.array_equal(ct, want_ct) and np.array_equal(ss, want_ss)
     assert np.array_equal(ss2, want_ss) and not fail.any()
+
+
+def test_serving_admission_by_hbm_budget(ctx, keys):
+    """The HBM budget bounds the device bytes of the key stores plus the
+    batches in flight, each batch reserving what its set, operation and size
+    need: with a budget of the stores plus ONE max_batch batch, smaller
+    batches of the four queues still run side by side; requests beyond what
+    fits wait (none fails) and give the plain calls' bytes; keys that would
+    leave less than one max_batch batch are refused."""
+    rng = np.random.default_rng(92)
+    sizing = T.Server(ctx, max_batch=4096, slots=4)
+    try:
+        for ps in SETS:
+            sizing.add_keys(ps, *keys[ps])
+        st0 = sizing.stats()
+    finally:
+        sizing.close()
+    budget = st0["store_bytes"] + st0["slot_bytes"]
+    srv = T.Server(ctx, hbm_budget=budget, max_batch=4096, slots=4)
+    reqs = []
+    try:
+        for ps in SETS:
+            srv.add_keys(ps, *keys[ps])
+        with pytest.raises(T.TmvpError) as e:  # more keys would not leave one batch
+            srv.add_keys("saber", *keys["saber"])
+        assert e.value.code == T.ENOMEM
+        outs, done = {}, {}
+        for k in range(24):
+            ps = SETS[k % 2]
+            op = T.SRV_ENCAPS if (k // 2) % 2 == 0 else T.SRV_DECAPS
+            n = int(rng.integers(600, 1400))
+            ids = rng.integers(0, NKEYS, n).astype(np.int32)
+            if op == T.SRV_ENCAPS:
+                data = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+            else:
+                data, _ = T.run_kem_encaps(ctx, keys[ps][0][ids], rng.integers(0, 256, (n, 32), dtype=np.uint8), ps,
+                                           T.MFMA)
+            reqs.append((op, ps, ids, data))
+        for k, (op, ps, ids, data) in enumerate(reqs):
+            _, outs[k] = srv.submit(op, ps, ids, data, tag=k)
+        while len(done) < len(reqs):
+            for c in srv.poll(64, timeout_us=200000):
+                done[c["tag"]] = c
+        assert all(c["status"] == 0 for c in done.values())
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["store_bytes"] == st0["store_bytes"] and st["hbm_budget"] == budget
+    assert st["max_inflight_bytes"] <= budget - st["store_bytes"]
+    assert st["max_concurrent"] >= 2  # a fixed slot count from this budget would allow one
+    for k in (0, 1, 2, 3, 22, 23):
+        op, ps, ids, data = reqs[k]
+        if op == T.SRV_ENCAPS:
+            want = T.run_kem_encaps(ctx, keys[ps][0][ids], data, ps, T.MFMA)
+        else:
+            want = T.run_kem_decaps(ctx, data, keys[ps][1][ids], ps, T.MFMA)
+        for got, exp in zip(outs[k], want):
+            assert np.array_equal(got, exp), (k, op, ps)
