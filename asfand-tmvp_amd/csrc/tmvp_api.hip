@@ -44,6 +44,7 @@ struct Workspace {
 constexpr int kMaxPlans = 32;
 struct Plan {
     uint64_t sig; // 0 = empty
+    int shape;    // MFMA tile: 16 or 32 (0 = the library's default)
     int wgs_per_cu;
     float weight[kNumKinds];
 };
@@ -110,6 +111,14 @@ inline int h1_of(const PS &p) { return 1 << (p.eq - p.ep - 1); }
 inline int h2_of(const PS &p) { return (1 << (p.ep - 2)) - (1 << (p.ep - p.et - 1)) + h1_of(p); }
 
 int hip_status(hipError_t e) { return e == hipSuccess ? TMVP_OK : TMVP_EHIP; }
+
+constexpr int kKnownFlags = TMVP_ACCUMULATE | TMVP_TILE_16 | TMVP_TILE_32;
+// The MFMA tile a call's flags name: 16, 32, 0 for none, -1 for both.
+int tile_of(int flags)
+{
+    const bool t16 = flags & TMVP_TILE_16, t32 = flags & TMVP_TILE_32;
+    return t16 && t32 ? -1 : t16 ? 16 : t32 ? 32 : 0;
+}
 
 int set_device(const tmvp_ctx *ctx)
 {
@@ -285,10 +294,11 @@ Job base_job(tmvp_ctx *ctx, uint16_t *out, const uint16_t *a, const uint16_t *s,
 // single-job entry point does.  Pointers are only required for batch > 0.
 int job_from_desc(tmvp_ctx *ctx, const tmvp_work_desc &d, Job &J, int &mode, int &l)
 {
-    if (d.batch < 0 || (d.flags & ~TMVP_ACCUMULATE)) return TMVP_EARG;
+    if (d.batch < 0 || (d.flags & ~kKnownFlags) || tile_of(d.flags) < 0) return TMVP_EARG;
     const int acc = (d.flags & TMVP_ACCUMULATE) ? 1 : 0;
     J = base_job(ctx, d.out, d.a, d.s, d.batch);
     J.accumulate = acc;
+    J.shape = tile_of(d.flags);
     bool need_aux = false;
     if (d.op == TMVP_OP_POLYMUL) {
         if (d.eq_bits < 1 || d.eq_bits > 16 || d.epilogue != 0) return TMVP_EARG;
@@ -351,6 +361,7 @@ struct WorkList {
     int kind[TMVP_MAX_WORK];
     int n;
     uint64_t sig;
+    int tile; // the tile the descriptors name (0: none)
 };
 
 int check_list(tmvp_ctx *ctx, int backend, const tmvp_work_desc *work, int n, WorkList &W, int *modes, int *ls)
@@ -358,11 +369,16 @@ int check_list(tmvp_ctx *ctx, int backend, const tmvp_work_desc *work, int n, Wo
     if (!ctx || n < 0 || n > TMVP_MAX_WORK || (n > 0 && !work)) return TMVP_EARG;
     W.n = 0;
     W.sig = 0xcbf29ce484222325ull;
+    W.tile = 0;
     for (int i = 0; i < n; i++) {
         Job J;
         int mode = 0, l = 0;
         const int rc = job_from_desc(ctx, work[i], J, mode, l);
         if (rc) return rc;
+        if (J.shape) {
+            if (W.tile && W.tile != J.shape) return TMVP_EARG; // one tile per launch
+            W.tile = J.shape;
+        }
         if (backend == TMVP_BACKEND_MFMA && (J.w < 8 || J.w > 14)) return TMVP_EUNSUPPORTED;
         if (J.batch == 0) continue;
         const int k = W.n++;
@@ -388,6 +404,7 @@ Plan default_plan(uint64_t sig)
 {
     Plan P;
     P.sig = sig;
+    P.shape = 0;      // the library's default tile
     P.wgs_per_cu = 0; // the kernel's full occupancy
     for (int k = 0; k < kNumKinds; k++) P.weight[k] = 1.0f;
     return P;
@@ -396,9 +413,11 @@ Plan default_plan(uint64_t sig)
 // Splits the resident workgroups between the jobs in proportion to
 // batch x per-instance cost x weight; each job gets at least one workgroup and
 // no more than it has groups of four instances.
+int plan_shape(const Plan &P) { return P.shape ? P.shape : mfma_default_shape(); }
+
 int build_multi(const tmvp_ctx *ctx, const WorkList &W, const Plan &P, MultiJob &M)
 {
-    int per_cu = mfma_multi_per_cu();
+    int per_cu = mfma_multi_per_cu(plan_shape(P));
     if (P.wgs_per_cu >= 1 && P.wgs_per_cu < per_cu) per_cu = P.wgs_per_cu;
     const double resident = (double)ctx->cus * per_cu;
     double total = 0.0;
@@ -427,8 +446,13 @@ int launch_list(tmvp_ctx *ctx, const WorkList &W, const Plan &P, hipStream_t st)
 {
     if (W.n == 0) return TMVP_OK;
     MultiJob M;
-    const int grid = build_multi(ctx, W, P, M);
-    return hip_status(launch_mfma_multi(M, grid, st));
+    Plan use = P;
+    if (W.tile && W.tile != plan_shape(P)) { // the descriptors name another tile than the plan
+        use.shape = W.tile;
+        use.wgs_per_cu = 0;
+    }
+    const int grid = build_multi(ctx, W, use, M);
+    return hip_status(launch_mfma_multi(M, grid, plan_shape(use), st));
 }
 
 } // namespace
@@ -548,24 +572,28 @@ int tmvp_ctx_status(tmvp_ctx *ctx, tmvp_stream_t stream)
 int tmvp_polymul(tmvp_ctx *ctx, int backend, uint16_t *d_c, const uint16_t *d_a, const uint16_t *d_s,
                  int64_t batch, int eq_bits, int flags, tmvp_stream_t stream)
 {
-    if (!ctx || batch < 0 || eq_bits < 1 || eq_bits > 16 || (flags & ~TMVP_ACCUMULATE)) return TMVP_EARG;
+    if (!ctx || batch < 0 || eq_bits < 1 || eq_bits > 16 || (flags & ~kKnownFlags) || tile_of(flags) < 0)
+        return TMVP_EARG;
     if (batch == 0) return TMVP_OK;
     if (!d_c || !d_a || !d_s) return TMVP_EARG;
     Job J = base_job(ctx, d_c, d_a, d_s, batch);
     J.w = eq_bits;
     J.accumulate = (flags & TMVP_ACCUMULATE) ? 1 : 0;
+    J.shape = tile_of(flags);
     return run(ctx, backend, J, kPoly, 1, stream);
 }
 
 int tmvp_matvec(tmvp_ctx *ctx, int ps, int backend, int transpose, uint16_t *d_out, const uint16_t *d_A,
                 const uint16_t *d_s, int64_t batch, int round_to_p, int flags, tmvp_stream_t stream)
 {
-    if (!ctx || ps < 0 || ps >= TMVP_NUM_PARAM_SETS || batch < 0 || (flags & ~TMVP_ACCUMULATE)) return TMVP_EARG;
+    if (!ctx || ps < 0 || ps >= TMVP_NUM_PARAM_SETS || batch < 0 || (flags & ~kKnownFlags) || tile_of(flags) < 0)
+        return TMVP_EARG;
     if (round_to_p && (flags & TMVP_ACCUMULATE)) return TMVP_EARG;
     if (batch == 0) return TMVP_OK;
     if (!d_out || !d_A || !d_s) return TMVP_EARG;
     const PS &p = kParams[ps];
     Job J = base_job(ctx, d_out, d_A, d_s, batch);
+    J.shape = tile_of(flags);
     J.w = p.eq;
     J.eq = p.eq;
     J.ep = p.ep;
@@ -581,13 +609,15 @@ int tmvp_matvec(tmvp_ctx *ctx, int ps, int backend, int transpose, uint16_t *d_o
 int tmvp_innerprod(tmvp_ctx *ctx, int ps, int backend, int epilogue, uint16_t *d_out, const uint16_t *d_b,
                    const uint16_t *d_s, const uint16_t *d_aux, int64_t batch, int flags, tmvp_stream_t stream)
 {
-    if (!ctx || ps < 0 || ps >= TMVP_NUM_PARAM_SETS || batch < 0 || (flags & ~TMVP_ACCUMULATE)) return TMVP_EARG;
+    if (!ctx || ps < 0 || ps >= TMVP_NUM_PARAM_SETS || batch < 0 || (flags & ~kKnownFlags) || tile_of(flags) < 0)
+        return TMVP_EARG;
     if (epilogue < TMVP_IP_RAW || epilogue > TMVP_IP_DEC) return TMVP_EARG;
     if (epilogue != TMVP_IP_RAW && (flags & TMVP_ACCUMULATE)) return TMVP_EARG;
     if (batch == 0) return TMVP_OK;
     if (!d_out || !d_b || !d_s || (epilogue != TMVP_IP_RAW && !d_aux)) return TMVP_EARG;
     const PS &p = kParams[ps];
     Job J = base_job(ctx, d_out, d_b, d_s, batch);
+    J.shape = tile_of(flags);
     J.w = p.ep;
     J.eq = p.eq;
     J.ep = p.ep;
@@ -736,17 +766,22 @@ int tmvp_batch_autotune(tmvp_ctx *ctx, const tmvp_work_desc *work, int n, tmvp_s
     Plan best = default_plan(W.sig);
     float best_t = 0.0f;
     rc = measure(best, best_t);
-    // workgroups per CU
-    for (int w = 1; !rc && w <= mfma_multi_per_cu(); w++) {
-        Plan c = best;
-        c.wgs_per_cu = w;
-        float t = 0.0f;
-        rc = measure(c, t);
-        if (!rc && t < best_t) {
-            best = c;
-            best_t = t;
+    // tile shape x workgroups per CU
+    const Plan base = best;
+    for (int shape : {16, 32}) {
+        for (int w = 1; !rc && w <= mfma_multi_per_cu(shape); w++) {
+            Plan c = base;
+            c.shape = shape;
+            c.wgs_per_cu = w;
+            float t = 0.0f;
+            rc = measure(c, t);
+            if (!rc && t < best_t) {
+                best = c;
+                best_t = t;
+            }
         }
     }
+    if (!best.shape) best.shape = mfma_default_shape(); // the plan records the tile it was measured with
     // per-kind weights, coordinate descent
     bool present[kNumKinds] = {};
     int kinds = 0;
@@ -784,13 +819,29 @@ int tmvp_batch_autotune(tmvp_ctx *ctx, const tmvp_work_desc *work, int n, tmvp_s
     return TMVP_OK;
 }
 
+int tmvp_batch_plan(tmvp_ctx *ctx, const tmvp_work_desc *work, int n, int *shape, int *wgs_per_cu)
+{
+    WorkList W;
+    int rc = check_list(ctx, TMVP_BACKEND_MFMA, work, n, W, nullptr, nullptr);
+    if (rc) return rc;
+    const Plan *P = find_plan(ctx, W.sig);
+    const Plan D = default_plan(W.sig);
+    const Plan &use = P ? *P : D;
+    int per_cu = mfma_multi_per_cu(plan_shape(use));
+    if (use.wgs_per_cu >= 1 && use.wgs_per_cu < per_cu) per_cu = use.wgs_per_cu;
+    if (shape) *shape = W.tile ? W.tile : plan_shape(use);
+    if (wgs_per_cu) *wgs_per_cu = per_cu;
+    return P ? TMVP_OK : 1;
+}
+
 int tmvp_graph_begin(tmvp_ctx *ctx, tmvp_stream_t stream)
 {
     if (!ctx || !stream) return TMVP_EARG;
     int rc = set_device(ctx);
     if (rc) return rc;
     // Resolve the launch-shape queries the kernels make on first use outside the capture.
-    (void)mfma_multi_per_cu();
+    (void)mfma_multi_per_cu(16);
+    (void)mfma_multi_per_cu(32);
     return hip_status(hipStreamBeginCapture(reinterpret_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal));
 }
 

@@ -42,6 +42,7 @@ struct Job {
     int accumulate;
     int h;               // h1 (q2p, enc) or h2 (dec)
     int eq, ep, et;
+    int shape;           // MFMA tile: 16 (16x16x64), 32 (32x32x32), 0 = the library's choice
 };
 
 __host__ __device__ inline constexpr int rows_of(int mode, int l)
@@ -91,8 +92,10 @@ struct MultiJob {
     Segment seg[kMaxSegments];
     int nseg;
 };
-// Workgroups of the multi-job kernel that fit on one CU.
-int mfma_multi_per_cu();
-hipError_t launch_mfma_multi(const MultiJob &M, int grid, hipStream_t stream);
+// The MFMA tile shape (16 or 32) launches use when a job or plan leaves it open.
+int mfma_default_shape();
+// Workgroups of the multi-job kernel of tile `shape` that fit on one CU.
+int mfma_multi_per_cu(int shape);
+hipError_t launch_mfma_multi(const MultiJob &M, int grid, int shape, hipStream_t stream);
 
 } // namespace tmvp

@@ -47,6 +47,18 @@
 // time, walks the batch with a grid stride, and loads each term of the next
 // instance into registers as soon as the current one has been staged.
 //
+// Second tile shape, v_mfma_i32_32x32x32_i8 (M=32, N=32, K=32), which issues
+// about 1.7x the int8 MACs per cycle of the 16x16x64 form on gfx950:
+//   M = the 32 rows i of a block row,
+//   K = the 32 jj of ONE D tile (eight MFMAs per row pair and term),
+//   N = 32 = 8 block columns R x 2 limbs x 2 public rows (a row pair rp;
+//       an odd row count leaves the second row of the last pair unused).
+// A-operand lane (i = lane & 31, kh = lane >> 5) holds the 16 band bytes
+// X[32D + i + 1 + 16kh ...] (K index k = 16kh + t is jj = 31 - k, which is
+// the byte order of the reversed E blocks), B-operand lane (n = lane & 31,
+// kh) the 16 bytes [16kh, 16kh + 16) of E[R - D + 8] of its limb and row.
+// The staging, the LDS layouts, the prefetch and the pipeline are shared.
+//
 // This header holds the device side (mfma_body and its helpers); the launches
 // (single job and mixed persistent) are in tmvp_mfma.hip.
 #pragma once
@@ -56,6 +68,7 @@
 namespace tmvp {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -127,6 +140,8 @@ template <int L, int MODE> struct Slice {
     static constexpr int NPOLY = NR * NT;       // public polynomials per instance
     static constexpr int CJ = NR * 32;          // 8-coefficient chunks of one term
     static constexpr int ITJ = (CJ + 63) / 64;  // staging loads per lane per term
+    // a lane past the last chunk stages chunk u0 - CJ, which must exist
+    static_assert(CJ % 64 == 0 || 64 * ITJ <= 2 * CJ, "chunk remap");
     static constexpr int E_BYTES = NR * 2 * kEPoly;
     static constexpr int BYTES = E_BYTES + kBandBytes;
 };
@@ -141,14 +156,18 @@ __device__ __forceinline__ uint32_t from_next_lane(uint32_t v)
 // instance; the A fragments are read one K-chunk ahead, which keeps even the
 // four-row FireSaber / Sable-3 shapes near 150 VGPRs.  The encryption shapes
 // (L + 1 rows) take 2, which they fill without spilling.
-template <int L, int MODE> constexpr int kWavesPerEU = (MODE == kEnc || rows_of(MODE, L) * terms_of(MODE, L) > 16) ? 2 : 3;
+// The 32x32x32 tile keeps two sets of 16 accumulators per row pair and takes
+// two waves per SIMD everywhere (three would spill).
+template <int L, int MODE, int SHAPE = 16>
+constexpr int kWavesPerEU = (SHAPE == 32 || MODE == kEnc || rows_of(MODE, L) * terms_of(MODE, L) > 16) ? 2 : 3;
 
 // LDS of one workgroup (four waves, two slices each).
 template <int L, int MODE> constexpr int kLdsBytes = 4 * 2 * Slice<L, MODE>::BYTES;
 
 // The work of one wave: instances first, first + stride, ... of job J, with
-// the wave's two LDS slices at lds + wave * 2 * Slice::BYTES.
-template <int L, int MODE, int EPI>
+// the wave's two LDS slices at lds + wave * 2 * Slice::BYTES.  SHAPE is the
+// MFMA tile: 16 (v_mfma_i32_16x16x64_i8) or 32 (v_mfma_i32_32x32x32_i8).
+template <int L, int MODE, int EPI, int SHAPE = 16>
 __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t stride, uint8_t *lds)
 {
     using S = Slice<L, MODE>;
@@ -169,14 +188,22 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     const int blk = (lane & 31) >> 2;     // staging: 32-coefficient block of the lane's chunk
     const int bpos = 24 - 8 * (lane & 3); // staging: byte position inside the reversed block
     const int il = lane & 15, g = lane >> 4;
+    // R (block column) and limb of the lane's MFMA column, in both shapes
     const int R = lane & 7, limb = (lane >> 3) & 1;
-    const int sh = (il + 1) & 3;
+    // 32x32x32: row i and K half of the A operand, row of the pair (rr) of the B operand
+    const int i32 = lane & 31, kh = lane >> 5, rr = (lane >> 4) & 1;
+    const int sh = SHAPE == 16 ? (il + 1) & 3 : (i32 + 1) & 3;
     // A operand: dword index of the window start x0 - sh in band copy `sh`, without
-    // the compile-time part 16p + 4h (x0 = 32D + 16h - 16(g&1) + il + 17, D = 2p + (g>>1);
-    // il + 17 - sh is a multiple of 4).
-    const int abase = band_copy(sh) / 4 + 8 * (g >> 1) - 4 * (g & 1) + ((il + 17 - sh) >> 2);
-    // B operand: byte offset of E[R - (g>>1) + 8] (minus 64 p per K-chunk) in the lane's limb plane
-    const int bbase = limb * kEPoly + 16 * (1 - (g & 1)) + (R - (g >> 1) + 8) * kEBlock;
+    // the compile-time part.  16: x0 = 32D + 16h - 16(g&1) + il + 17 with
+    // D = 2p + (g>>1), part 16p + 4h; 32: x0 = 32D + i + 1 + 16kh, part 8D.
+    // (x0 - sh is a multiple of 4.)
+    const int abase = SHAPE == 16 ? band_copy(sh) / 4 + 8 * (g >> 1) - 4 * (g & 1) + ((il + 17 - sh) >> 2)
+                                  : band_copy(sh) / 4 + 4 * kh + ((i32 + 1 - sh) >> 2);
+    // B operand: byte offset of the lane's E block in its limb plane (and row),
+    // 16: E[R - (g>>1) + 8], minus 64 p per K-chunk; 32: E[R + 8] of row rr of
+    // the pair, minus 32 D per tile, plus 2 kEPoly per row.
+    const int bbase = SHAPE == 16 ? limb * kEPoly + 16 * (1 - (g & 1)) + (R - (g >> 1) + 8) * kEBlock
+                                  : rr * 2 * kEPoly + limb * kEPoly + 16 * kh + (R + 8) * kEBlock;
 
     // Registers holding one instance's inputs, term by term.
     u32x4 va[NT][ITJ];
@@ -198,7 +225,7 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
 #pragma unroll
         for (int it = 0; it < ITJ; it++) {
             const int u0 = lane + 64 * it;                             // chunk of this term: row u >> 5, piece u & 31
-            const int u = (CJ % 64 == 0 || u0 < CJ) ? u0 : u0 - 64;    // always a chunk of this term
+            const int u = (CJ % 64 == 0 || u0 < CJ) ? u0 : u0 - CJ;    // always a chunk of this term
             const int r = u >> 5;
             if constexpr (MODE == kEnc) {
                 const u32x4 *row = r < L ? src + (r * L + j) * 32
@@ -222,6 +249,10 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
         // ---- public polynomials of term j -> byte-reversed limb blocks E[r][limb][e]
 #pragma unroll
         for (int it = 0; it < ITJ; it++) {
+            // Lanes past the last chunk skip the staging.  (A branch-free form,
+            // where they rewrite a copy of another lane's chunk, lets the
+            // scheduler mix the staging, which waits for the prefetched loads,
+            // into the MFMAs, and measured 4 us slower at B = 65536.)
             const int u = lane + 64 * it;
             if (CJ % 64 != 0 && u >= CJ) continue;
             const int r = u >> 5;
@@ -249,17 +280,21 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
         // Band dword k: k = lane + 8 is `pos` of this lane, k = lane - 56 (lanes
         // 56..63) is `neg`; dword 72 is zero padding.
         const uint32_t p1 = from_next_lane(pos);
-        uint32_t n1 = from_next_lane(neg);
-        if (lane == 63) n1 = __builtin_amdgcn_readlane(pos, 0); // dword 8 follows dword 7
+        // n1: the next lane's neg, and for lane 63 (dword 8 follows dword 7)
+        // lane 0's pos.  One DPP move whose out-of-range lane keeps the old
+        // value: a select here was compiled into a branch around the DPP move,
+        // whose source lane 63 was then disabled and read as 0.
+        const uint32_t n1 =
+            (uint32_t)__builtin_amdgcn_update_dpp((int)__builtin_amdgcn_readlane(pos, 0), (int)neg, 0x130, 0xf, 0xf, false);
         uint8_t *band = X + 32 + 4 * lane;
         *(uint32_t *)band = pos;
 #pragma unroll
         for (int c = 1; c < 4; c++) *(uint32_t *)(band + band_copy(c)) = __builtin_amdgcn_alignbyte(p1, pos, c);
+        // The wrapped dwords 0..7 come from lanes 56..63.
         if (lane >= 56) {
             uint8_t *nb = X + 4 * lane - 224;
-            *(uint32_t *)nb = neg;
 #pragma unroll
-            for (int c = 1; c < 4; c++) *(uint32_t *)(nb + band_copy(c)) = __builtin_amdgcn_alignbyte(n1, neg, c);
+            for (int c = 0; c < 4; c++) *(uint32_t *)(nb + band_copy(c)) = c ? __builtin_amdgcn_alignbyte(n1, neg, c) : neg;
         }
     };
     // The slices are private to this wave and the LDS completes one wave's
@@ -291,115 +326,221 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     // the epilogue never adds it.
     const int hinit = (EPI != kEpiMod && limb == 0) ? J.h : 0;
     const v4i init = {hinit, hinit, hinit, hinit};
+    v16i init16;
+#pragma unroll
+    for (int e = 0; e < 16; e++) init16[e] = hinit;
     // Epilogue roles: lo-limb lanes finish rows v = 0, 1 of their group, hi-limb
     // lanes rows v = 2, 3; each sends the partner the other pair.
     const uint32_t lshift = limb ? 0x00070007u : 0u;
     for (;;) {
         const int64_t next = inst + stride;
-        v4i acc[NR][2];
-#pragma unroll
-        for (int j = 0; j < NT; j++) {
-            const uint8_t *const E = slice0 + buf * S::BYTES;
-            // A operand: Toeplitz tiles of s_j for the four K-chunks and both halves,
-            // 16 bytes from a dword-aligned window each.  The window's byte offset is
-            // made opaque so that the 16 reads share one address register and carry
-            // their constant parts as instruction offsets.
-            int aoff = buf * S::BYTES + S::E_BYTES + 4 * abase;
-            asm volatile("" : "+v"(aoff));
-            const uint32_t *aw = reinterpret_cast<const uint32_t *>(slice0 + aoff);
-            // A fragments of one K-chunk (both halves); chunk p + 1 is read while
-            // chunk p is multiplied.
-            auto read_a = [&](int p, v4i (&f)[2]) {
-#pragma unroll
+        if constexpr (SHAPE == 16) {
+            v4i acc[NR][2];
+    #pragma unroll
+            for (int j = 0; j < NT; j++) {
+                const uint8_t *const E = slice0 + buf * S::BYTES;
+                // A operand: Toeplitz tiles of s_j for the four K-chunks and both halves,
+                // 16 bytes from a dword-aligned window each.  The window's byte offset is
+                // made opaque so that the 16 reads share one address register and carry
+                // their constant parts as instruction offsets.
+                int aoff = buf * S::BYTES + S::E_BYTES + 4 * abase;
+                asm volatile("" : "+v"(aoff));
+                const uint32_t *aw = reinterpret_cast<const uint32_t *>(slice0 + aoff);
+                // A fragments of one K-chunk (both halves); chunk p + 1 is read while
+                // chunk p is multiplied.
+                auto read_a = [&](int p, v4i (&f)[2]) {
+    #pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const uint32_t *w = aw + 16 * p + 4 * h;
+                        f[h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+                    }
+                };
+                v4i acur[2];
+                read_a(0, acur);
+                // B operand of K-chunk 0; chunk p + 1 is read while chunk p is multiplied.
+                v4i bcur[NR];
+    #pragma unroll
+                for (int r = 0; r < NR; r++) bcur[r] = *(const v4i *)(E + r * 2 * kEPoly + bbase);
+
+                if (j + 1 < NT) {
+                    stage(j + 1, buf ^ 1);
+                    pin_after_stage();
+                    fetch_term(next, j + 1);
+                } else if (next < J.batch) {
+                    stage(0, buf ^ 1); // va[0] / vs[0] already hold the next instance's term 0
+                    pin_after_stage();
+                    fetch_term(next + stride, 0);
+                }
+
+                // K-chunk outermost: 2*NR independent accumulators between dependent MFMAs.
+    #pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    v4i bnxt[NR], anxt[2];
+                    if (p < 3) {
+                        read_a(p + 1, anxt);
+    #pragma unroll
+                        for (int r = 0; r < NR; r++) bnxt[r] = *(const v4i *)(E + r * 2 * kEPoly + bbase - 64 * (p + 1));
+                    }
+    #pragma unroll
+                    for (int r = 0; r < NR; r++) {
+                        const bool first = (j == 0 && p == 0);
+                        acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur[0], bcur[r], first ? init : acc[r][0], 0, 0, 0);
+                        acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur[1], bcur[r], first ? init : acc[r][1], 0, 0, 0);
+                    }
+                    if (p < 3) {
+                        acur[0] = anxt[0];
+                        acur[1] = anxt[1];
+    #pragma unroll
+                        for (int r = 0; r < NR; r++) bcur[r] = bnxt[r];
+                    }
+                }
+                // Schedule: chunk 0's fragment reads first, then per chunk the next
+                // chunk's A and B reads and one MFMA per three staging VALU instructions.
+                __builtin_amdgcn_sched_group_barrier(0x100, 4 + NR, 0);
+    #pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    if (p < 3) __builtin_amdgcn_sched_group_barrier(0x100, 4 + NR, 0);
+    #pragma unroll
+                    for (int k = 0; k < 2 * NR; k++) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                        __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+                    }
+                }
+                buf ^= 1;
+                wave_fence();
+            }
+
+            // ---- epilogue, in packed 16-bit arithmetic (every output is mod 2^w,
+            //      w <= 14): lane (R, limb, g) holds rows i = 16h + 4g + v of block
+            //      column R for its limb.  Both lanes of a pair scale their own limb
+            //      (hi by 128), keep one row pair and send the other across
+            //      row_ror:8, so each finishes two rows: lo lanes v = 0,1, hi v = 2,3.
+            //      kEnc: rows 0..L-1 go to J.out (q->p), row L to J.out2 (enc).
+            constexpr int NRO = (MODE == kEnc) ? L : NR; // rows in J.out per instance
+            const int lpos = 32 * R + 4 * g + 2 * limb;
+            uint16_t *obase = J.out + inst * (NRO * kN) + lpos;
+            const bool with_aux = EPI == kEpiEnc || EPI == kEpiDec || MODE == kEnc;
+            const uint16_t *xbase = with_aux ? J.aux + inst * kN + lpos : nullptr;
+    #pragma unroll
+            for (int r = 0; r < NR; r++) {
+    #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    const uint32_t *w = aw + 16 * p + 4 * h;
-                    f[h] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-                }
-            };
-            v4i acur[2];
-            read_a(0, acur);
-            // B operand of K-chunk 0; chunk p + 1 is read while chunk p is multiplied.
-            v4i bcur[NR];
-#pragma unroll
-            for (int r = 0; r < NR; r++) bcur[r] = *(const v4i *)(E + r * 2 * kEPoly + bbase);
-
-            if (j + 1 < NT) {
-                stage(j + 1, buf ^ 1);
-                pin_after_stage();
-                fetch_term(next, j + 1);
-            } else if (next < J.batch) {
-                stage(0, buf ^ 1); // va[0] / vs[0] already hold the next instance's term 0
-                pin_after_stage();
-                fetch_term(next + stride, 0);
-            }
-
-            // K-chunk outermost: 2*NR independent accumulators between dependent MFMAs.
-#pragma unroll
-            for (int p = 0; p < 4; p++) {
-                v4i bnxt[NR], anxt[2];
-                if (p < 3) {
-                    read_a(p + 1, anxt);
-#pragma unroll
-                    for (int r = 0; r < NR; r++) bnxt[r] = *(const v4i *)(E + r * 2 * kEPoly + bbase - 64 * (p + 1));
-                }
-#pragma unroll
-                for (int r = 0; r < NR; r++) {
-                    const bool first = (j == 0 && p == 0);
-                    acc[r][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur[0], bcur[r], first ? init : acc[r][0], 0, 0, 0);
-                    acc[r][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(acur[1], bcur[r], first ? init : acc[r][1], 0, 0, 0);
-                }
-                if (p < 3) {
-                    acur[0] = anxt[0];
-                    acur[1] = anxt[1];
-#pragma unroll
-                    for (int r = 0; r < NR; r++) bcur[r] = bnxt[r];
+                    const uint32_t p01 = shl16x2(perm((uint32_t)acc[r][h][1], (uint32_t)acc[r][h][0], 0x05040100u), lshift);
+                    const uint32_t p23 = shl16x2(perm((uint32_t)acc[r][h][3], (uint32_t)acc[r][h][2], 0x05040100u), lshift);
+                    const uint32_t keep = limb ? p23 : p01;
+                    const uint32_t send = limb ? p01 : p23;
+                    const uint32_t y = add16x2(keep, (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0x128, 0xf, 0xf, false));
+                    if (MODE == kEnc && r == L) {
+                        const uint32_t auxv = *(const uint32_t *)(xbase + 16 * h);
+                        *(uint32_t *)(J.out2 + inst * kN + lpos + 16 * h) = epi2<kEpiEnc>(J, y, 0u, auxv);
+                        continue;
+                    }
+                    uint32_t old = 0u, auxv = 0u;
+                    if (EPI == kEpiMod && J.accumulate) old = *(const uint32_t *)(obase + r * kN + 16 * h);
+                    if (EPI == kEpiEnc || EPI == kEpiDec) auxv = *(const uint32_t *)(xbase + 16 * h);
+                    *(uint32_t *)(obase + r * kN + 16 * h) = epi2<EPI>(J, y, old, auxv);
                 }
             }
-            // Schedule: chunk 0's fragment reads first, then per chunk the next
-            // chunk's A and B reads and one MFMA per three staging VALU instructions.
-            __builtin_amdgcn_sched_group_barrier(0x100, 4 + NR, 0);
+        } else {
+            // ---- 32x32x32 tiles: per term, D = 0..7, one A fragment per tile shared
+            //      by the NP row pairs, one B fragment per (tile, pair)
+            constexpr int NP = (NR + 1) / 2;
+            // two accumulator sets (even and odd D tiles), summed in the
+            // epilogue: the same accumulator then recurs every 2 NP MFMAs
+            v16i acc[2][NP];
 #pragma unroll
-            for (int p = 0; p < 4; p++) {
-                if (p < 3) __builtin_amdgcn_sched_group_barrier(0x100, 4 + NR, 0);
+            for (int j = 0; j < NT; j++) {
+                const uint8_t *const E = slice0 + buf * S::BYTES;
+                int aoff = buf * S::BYTES + S::E_BYTES + 4 * abase;
+                asm volatile("" : "+v"(aoff));
+                const uint32_t *aw = reinterpret_cast<const uint32_t *>(slice0 + aoff);
+                auto read_a = [&](int D) {
+                    const uint32_t *w = aw + 8 * D;
+                    return v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+                };
+                auto read_b = [&](int D, int rp) { return *(const v4i *)(E + bbase + rp * 4 * kEPoly - kEBlock * D); };
+                v4i acur = read_a(0);
+                v4i bcur[NP];
 #pragma unroll
-                for (int k = 0; k < 2 * NR; k++) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
-                }
-            }
-            buf ^= 1;
-            wave_fence();
-        }
+                for (int rp = 0; rp < NP; rp++) bcur[rp] = read_b(0, rp);
 
-        // ---- epilogue, in packed 16-bit arithmetic (every output is mod 2^w,
-        //      w <= 14): lane (R, limb, g) holds rows i = 16h + 4g + v of block
-        //      column R for its limb.  Both lanes of a pair scale their own limb
-        //      (hi by 128), keep one row pair and send the other across
-        //      row_ror:8, so each finishes two rows: lo lanes v = 0,1, hi v = 2,3.
-        //      kEnc: rows 0..L-1 go to J.out (q->p), row L to J.out2 (enc).
-        constexpr int NRO = (MODE == kEnc) ? L : NR; // rows in J.out per instance
-        const int lpos = 32 * R + 4 * g + 2 * limb;
-        uint16_t *obase = J.out + inst * (NRO * kN) + lpos;
-        const bool with_aux = EPI == kEpiEnc || EPI == kEpiDec || MODE == kEnc;
-        const uint16_t *xbase = with_aux ? J.aux + inst * kN + lpos : nullptr;
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const uint32_t p01 = shl16x2(perm((uint32_t)acc[r][h][1], (uint32_t)acc[r][h][0], 0x05040100u), lshift);
-                const uint32_t p23 = shl16x2(perm((uint32_t)acc[r][h][3], (uint32_t)acc[r][h][2], 0x05040100u), lshift);
-                const uint32_t keep = limb ? p23 : p01;
-                const uint32_t send = limb ? p01 : p23;
-                const uint32_t y = add16x2(keep, (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0x128, 0xf, 0xf, false));
-                if (MODE == kEnc && r == L) {
-                    const uint32_t auxv = *(const uint32_t *)(xbase + 16 * h);
-                    *(uint32_t *)(J.out2 + inst * kN + lpos + 16 * h) = epi2<kEpiEnc>(J, y, 0u, auxv);
-                    continue;
+                if (j + 1 < NT) {
+                    stage(j + 1, buf ^ 1);
+                    pin_after_stage();
+                    fetch_term(next, j + 1);
+                } else if (next < J.batch) {
+                    stage(0, buf ^ 1); // va[0] / vs[0] already hold the next instance's term 0
+                    pin_after_stage();
+                    fetch_term(next + stride, 0);
                 }
-                uint32_t old = 0u, auxv = 0u;
-                if (EPI == kEpiMod && J.accumulate) old = *(const uint32_t *)(obase + r * kN + 16 * h);
-                if (EPI == kEpiEnc || EPI == kEpiDec) auxv = *(const uint32_t *)(xbase + 16 * h);
-                *(uint32_t *)(obase + r * kN + 16 * h) = epi2<EPI>(J, y, old, auxv);
+
+                // tile D + 1's fragments are read while tile D is multiplied
+#pragma unroll
+                for (int D = 0; D < 8; D++) {
+                    v4i anxt, bnxt[NP];
+                    if (D < 7) {
+                        anxt = read_a(D + 1);
+#pragma unroll
+                        for (int rp = 0; rp < NP; rp++) bnxt[rp] = read_b(D + 1, rp);
+                    }
+#pragma unroll
+                    for (int rp = 0; rp < NP; rp++) {
+                        const v16i c0 = (D == 0) ? init16 : v16i{};
+                        acc[D & 1][rp] = __builtin_amdgcn_mfma_i32_32x32x32_i8(acur, bcur[rp],
+                                                                               (j == 0 && D < 2) ? c0 : acc[D & 1][rp], 0, 0, 0);
+                    }
+                    if (D < 7) {
+                        acur = anxt;
+#pragma unroll
+                        for (int rp = 0; rp < NP; rp++) bcur[rp] = bnxt[rp];
+                    }
+                }
+                // Schedule: tile 0's fragment reads first, then per tile the next
+                // tile's reads and one MFMA per five staging VALU instructions.
+                __builtin_amdgcn_sched_group_barrier(0x100, 2 + NP, 0);
+#pragma unroll
+                for (int D = 0; D < 8; D++) {
+                    if (D < 7) __builtin_amdgcn_sched_group_barrier(0x100, 2 + NP, 0);
+#pragma unroll
+                    for (int k = 0; k < NP; k++) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                        __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+                    }
+                }
+                buf ^= 1;
+                wave_fence();
+            }
+
+            // ---- epilogue: lane (R, limb, rr, kh) of pair rp holds rows
+            //      i = 8q + 4kh + v (v = 0..3) in accumulators 4q + v, of public row
+            //      2rp + rr.  As in the 16x16 form, each lane scales its limb, keeps
+            //      one row pair of every group and sends the other across row_ror:8.
+            constexpr int NRO = (MODE == kEnc) ? L : NR; // rows in J.out per instance
+#pragma unroll
+            for (int rp = 0; rp < NP; rp++) {
+                const int r = 2 * rp + rr;
+                const v16i sum = acc[0][rp] + acc[1][rp];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t p01 =
+                        shl16x2(perm((uint32_t)sum[4 * q + 1], (uint32_t)sum[4 * q], 0x05040100u), lshift);
+                    const uint32_t p23 =
+                        shl16x2(perm((uint32_t)sum[4 * q + 3], (uint32_t)sum[4 * q + 2], 0x05040100u), lshift);
+                    const uint32_t keep = limb ? p23 : p01;
+                    const uint32_t send = limb ? p01 : p23;
+                    const uint32_t y = add16x2(keep, (uint32_t)__builtin_amdgcn_mov_dpp((int)send, 0x128, 0xf, 0xf, false));
+                    if (r >= NR) continue; // the unused row of an odd row count
+                    const int pos = 32 * R + 8 * q + 4 * kh + 2 * limb;
+                    const bool need_aux = MODE == kEnc ? r == L : (EPI == kEpiEnc || EPI == kEpiDec);
+                    const uint32_t auxv = need_aux ? *(const uint32_t *)(J.aux + inst * kN + pos) : 0u;
+                    if (MODE == kEnc && r == L) {
+                        *(uint32_t *)(J.out2 + inst * kN + pos) = epi2<kEpiEnc>(J, y, 0u, auxv);
+                        continue;
+                    }
+                    uint16_t *o = J.out + inst * (NRO * kN) + r * kN + pos;
+                    const uint32_t old = (EPI == kEpiMod && J.accumulate) ? *(const uint32_t *)o : 0u;
+                    *(uint32_t *)o = epi2<EPI>(J, y, old, auxv);
+                }
             }
         }
         inst = next;

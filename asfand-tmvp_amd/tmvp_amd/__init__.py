@@ -32,7 +32,18 @@ PARAM_SET_NAMES = {"lightsaber": 0, "saber": 1, "firesaber": 2, "sable1": 3, "sa
 # backends: reference mode 0 -> MFMA, 1 -> VALU, 2 -> schoolbook (VALU)
 MFMA, VALU, SCHOOLBOOK = 0, 1, 2
 ACCUMULATE = 0x1
+# MFMA tile flags (include/tmvp.h)
+TILE_16, TILE_32 = 0x2, 0x4
+# Wrapper convenience for the operator calls below: the MFMA backend with the
+# 32x32x32 int8 tile (passed to the library as MFMA + TILE_32).
+MFMA32 = 16
+
+
 IP_RAW, IP_ENC, IP_DEC = 0, 1, 2
+
+
+def _backend_flags(backend, flags):
+    return (MFMA, flags | TILE_32) if backend == MFMA32 else (backend, flags)
 
 # status codes (include/tmvp.h)
 OK, EARG, EHIP, ENOMEM, EUNSUPPORTED, ERANGE, ENODEV = 0, -1, -2, -3, -4, -5, -6
@@ -69,7 +80,7 @@ EXPORTS = ("tmvp_init", "tmvp_destroy", "tmvp_ctx_status", "tmvp_strerror", "tmv
            "tmvp_kem_keypair", "tmvp_kem_encaps", "tmvp_kem_decaps",
            "tmvp_host_configure", "tmvp_host_set_randomness", "tmvp_host_status", "tmvp_host_release",
            "saber_enc", "saber_dec", "SABLE_enc", "SABLE_dec",
-           "tmvp_batch_launch", "tmvp_batch_autotune", "tmvp_graph_begin", "tmvp_graph_end", "tmvp_graph_launch",
+           "tmvp_batch_launch", "tmvp_batch_autotune", "tmvp_batch_plan", "tmvp_graph_begin", "tmvp_graph_end", "tmvp_graph_launch",
            "tmvp_graph_destroy", "tmvp_kem_reserve", "tmvp_kem_encaps_noalloc",
            "tmvp_srv_create", "tmvp_srv_add_keys", "tmvp_srv_submit", "tmvp_srv_poll", "tmvp_srv_drain",
            "tmvp_srv_get_stats", "tmvp_srv_destroy")
@@ -144,6 +155,7 @@ def load_library(path=LIB_PATH):
         "SABLE_dec": (None, [P, P, P, P, i]),
         "tmvp_batch_launch": (i, [P, i, P, i, P]),
         "tmvp_batch_autotune": (i, [P, P, i, P, ctypes.POINTER(ctypes.c_float)]),
+        "tmvp_batch_plan": (i, [P, P, i, ctypes.POINTER(i), ctypes.POINTER(i)]),
         "tmvp_graph_begin": (i, [P, P]),
         "tmvp_graph_end": (i, [P, P, ctypes.POINTER(P)]),
         "tmvp_graph_launch": (i, [P, P]),
@@ -359,6 +371,7 @@ class Context:
             raise TmvpError(rc, "tmvp_ctx_status")
 
     def polymul(self, c, a, s, batch, eq_bits=13, backend=MFMA, flags=0, stream=None):
+        backend, flags = _backend_flags(backend, flags)
         rc = load_library().tmvp_polymul(self.handle, backend, _ptr(c), _ptr(a), _ptr(s), int(batch),
                                          int(eq_bits), int(flags), _stream(stream))
         if rc:
@@ -366,6 +379,7 @@ class Context:
 
     def matvec(self, out, A, s, batch, param_set=SABER, backend=MFMA, transpose=False, round_to_p=False,
                flags=0, stream=None):
+        backend, flags = _backend_flags(backend, flags)
         ps = PARAM_SET_NAMES.get(param_set, param_set)
         rc = load_library().tmvp_matvec(self.handle, int(ps), backend, int(bool(transpose)), _ptr(out), _ptr(A),
                                         _ptr(s), int(batch), int(bool(round_to_p)), int(flags), _stream(stream))
@@ -374,6 +388,7 @@ class Context:
 
     def innerprod(self, out, b, s, batch, param_set=SABER, backend=MFMA, epilogue=IP_RAW, aux=None, flags=0,
                   stream=None):
+        backend, flags = _backend_flags(backend, flags)
         ps = PARAM_SET_NAMES.get(param_set, param_set)
         rc = load_library().tmvp_innerprod(self.handle, int(ps), backend, int(epilogue), _ptr(out), _ptr(b),
                                            _ptr(s), _ptr(aux), int(batch), int(flags), _stream(stream))
@@ -418,6 +433,16 @@ class Context:
         if rc:
             raise TmvpError(rc, "tmvp_batch_autotune")
         return ms.value
+
+    def batch_plan(self, descs):
+        """(tile shape, workgroups per CU, measured) of the plan this list uses."""
+        arr = (WorkDesc * len(descs))(*descs)
+        shape, wgs = ctypes.c_int(), ctypes.c_int()
+        rc = load_library().tmvp_batch_plan(self.handle, ctypes.cast(arr, ctypes.c_void_p), len(descs),
+                                            ctypes.byref(shape), ctypes.byref(wgs))
+        if rc < 0:
+            raise TmvpError(rc, "tmvp_batch_plan")
+        return shape.value, wgs.value, rc == 0
 
     def kem_reserve(self, param_set, batch, stream=None):
         ps = PARAM_SET_NAMES.get(param_set, param_set)

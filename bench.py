@@ -380,6 +380,7 @@ def mixed_bench(ctx, stream, batch=4096, steps=50):
     stream.synchronize()
     same = all(np.array_equal(a, x.download(np.uint16)) for a, x in zip(ref, bufs[3::5] + bufs[4::5]))
     tuned = ctx.batch_autotune(descs, stream=stream)
+    tile, wgs, _ = ctx.batch_plan(descs)
     ms_tuned = time_launches(fused, stream, steps, 5) / steps
     g = ctx.capture(stream, fused)
     ms_graph = time_launches(lambda: g.launch(stream), stream, steps, 5) / steps
@@ -388,7 +389,8 @@ def mixed_bench(ctx, stream, batch=4096, steps=50):
     best = min(ms_def, ms_tuned, ms_graph)
     out.update({
         "separate_launches_ms": ms_sep, "persistent_default_ms": ms_def, "persistent_autotuned_ms": ms_tuned,
-        "autotune_best_ms": tuned, "graph_replay_ms": ms_graph,
+        "autotune_best_ms": tuned, "autotuned_plan": {"mfma_tile": tile, "wgs_per_cu": wgs},
+        "graph_replay_ms": ms_graph,
         "instances_per_s": 3 * batch / (best / 1e3),
         "achieved_GBs": total_bytes / (best / 1e3) / 1e9,
         "hbm_frac": total_bytes / (best / 1e3) / 1e9 / HBM_PEAK_GBS,
@@ -606,6 +608,10 @@ def main():
                     help="offered request rates, comma separated (a sweep up to past saturation)")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="seconds of arrivals per rate, capped at 36000 requests per rate")
+    ap.add_argument("--only-kem", action="store_true",
+                    help="only the batched KEM leg (no CPU rate): for profiler runs")
+    ap.add_argument("--only-matvec", action="store_true",
+                    help="only the headline MFMA MatVec timing (cold, settle, settled): for profiler runs")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run device-to-device copies of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -617,6 +623,13 @@ def main():
 
     if args.calib_copy_mib:
         calib_copy(args.calib_copy_mib)
+        return
+    if args.only_kem:
+        ctx = T.Context(0)
+        stream = T.Stream()
+        print(json.dumps({"kem": kem_bench(ctx, stream, args.kem_batch, args.kem_reps, with_cpu=False)}))
+        stream.destroy()
+        ctx.close()
         return
     print(f"[bench] Saber MatVec B={args.batch} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
     ctx = T.Context(0)
@@ -651,6 +664,13 @@ def main():
     ms_mfma = time_launches(run_mfma, stream, args.steps, 0)
     print(f"[bench] mfma {ms_mfma / args.steps * 1e3:.1f} us/launch after {args.settle} settle launches",
           file=sys.stderr, flush=True)
+    if args.only_matvec:
+        per_launch_s = ms_mfma / args.steps / 1e3
+        print(json.dumps({"metric": "Saber MatVec TMVP ops/s at B=65536 (MFMA only)", "value": args.batch / per_launch_s,
+                          "unit": "MatVec/s", "ms_per_step": ms_mfma / args.steps,
+                          "ms_per_step_cold": ms_cold / args.steps, "steps": args.steps, "warmup": args.warmup,
+                          "settle_launches": args.settle}))
+        return
     # the same launch captured in a graph: median of 25 replays, each between
     # its own pair of events (so each pays its own launch and ramp-up)
     g = ctx.capture(stream, run_mfma)

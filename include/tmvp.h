@@ -79,6 +79,13 @@ enum tmvp_backend {
 
 /* ---- flags ---- */
 #define TMVP_ACCUMULATE 0x1 /* out = (out + result) mod 2^W, the reference's `r += sum` */
+/* MFMA backend only: the int8 matrix-core tile, v_mfma_i32_16x16x64_i8 or
+ * v_mfma_i32_32x32x32_i8 (results are the same).  Without either flag the
+ * library's default tile is used (the 16x16x64 one, which measures faster
+ * for every shape here); in a work list every descriptor that names a tile
+ * must name the same one, and it overrides the plan's. */
+#define TMVP_TILE_16 0x2
+#define TMVP_TILE_32 0x4
 
 /* ---- InnerProd epilogues ---- */
 enum tmvp_ip_epilogue {
@@ -179,8 +186,16 @@ int tmvp_batch_launch(tmvp_ctx *ctx, int backend, const tmvp_work_desc *work, in
 /* Times candidate plans for this work list on its own buffers (outputs are
  * overwritten; TMVP_ACCUMULATE is refused), keeps the fastest in the context
  * for later launches of lists of the same shape, and returns its time per
- * launch in *best_ms (may be NULL).  Synchronises `stream`. */
+ * launch in *best_ms (may be NULL).  The candidates cover the MFMA tile shape
+ * (16x16x64 or 32x32x32 int8 tiles), the workgroups per CU and the split
+ * between the jobs.  Synchronises `stream`. */
 int tmvp_batch_autotune(tmvp_ctx *ctx, const tmvp_work_desc *work, int n, tmvp_stream_t stream, float *best_ms);
+
+/* The plan launches of this work list's shape use: the MFMA tile shape
+ * (16 = v_mfma_i32_16x16x64_i8 tiles, 32 = v_mfma_i32_32x32x32_i8 tiles) and
+ * the workgroups per CU.  Returns TMVP_OK for a plan tmvp_batch_autotune
+ * measured, 1 for the default plan (cost-model split, full occupancy). */
+int tmvp_batch_plan(tmvp_ctx *ctx, const tmvp_work_desc *work, int n, int *shape, int *wgs_per_cu);
 
 /* ---- hipGraph capture of a sequence of library calls on one stream ----
  * Between tmvp_graph_begin and tmvp_graph_end every tmvp call on `stream` is

@@ -151,13 +151,40 @@ def test_mixed_full_size_autotune(ctx):
     first = [j.got() for j in jobs]
     for j, g in zip(jobs, first):
         assert np.array_equal(g, j.want), (j.op, j.ps)
-    ms = ctx.batch_autotune([j.desc for j in jobs])
+    descs = [j.desc for j in jobs]
+    shape, wgs, measured = ctx.batch_plan(descs)
+    assert not measured and shape in (16, 32) and wgs >= 1
+    ms = ctx.batch_autotune(descs)
     assert 0.0 < ms < 50.0
+    shape, wgs, measured = ctx.batch_plan(descs)  # the plan records the tile it was measured with
+    assert measured and shape in (16, 32) and wgs >= 1
     for j in jobs:
         j.dout.zero()
     run_list(ctx, jobs)
     for j, g in zip(jobs, first):
         assert np.array_equal(j.got(), g), (j.op, j.ps)
+        j.free()
+
+
+@pytest.mark.parametrize("tile", [T.TILE_16, T.TILE_32], ids=["tile16", "tile32"])
+def test_mixed_launch_each_tile(ctx, tile):
+    """Every kind in one launch with the descriptors naming the MFMA tile: the
+    16x16x64 and the 32x32x32 multi-job kernels both equal the oracle; the
+    plan query reports the tile, and descriptors naming two tiles are refused."""
+    rng = np.random.default_rng(70 + tile)
+    jobs = config3_jobs(rng, 97)
+    for j in jobs:
+        j.desc.flags |= tile
+    run_list(ctx, jobs)
+    for j in jobs:
+        assert np.array_equal(j.got(), j.want), (j.op, j.ps, tile)
+    shape, _, _ = ctx.batch_plan([j.desc for j in jobs])
+    assert shape == (32 if tile == T.TILE_32 else 16)
+    jobs[0].desc.flags ^= T.TILE_16 | T.TILE_32
+    with pytest.raises(T.TmvpError) as e:
+        ctx.batch_launch([j.desc for j in jobs])
+    assert e.value.code == T.EARG
+    for j in jobs:
         j.free()
 
 

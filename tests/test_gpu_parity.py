@@ -19,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 N = 256
 PS_ALL = ["lightsaber", "saber", "firesaber", "sable1", "sable2", "sable3"]
-BACKENDS = [T.MFMA, T.VALU]
+# the MFMA form with each int8 tile (16x16x64 by default, 32x32x32 via TMVP_TILE_32) and the VALU form
+BACKENDS = [T.MFMA, T.MFMA32, T.VALU]
 
 
 @pytest.fixture(scope="module")
@@ -227,6 +228,7 @@ def test_saber_matvec_full_batch_vs_oracle(ctx):
     mf = T.run_matvec(ctx, A, s, "saber", T.MFMA, round_to_p=True)
     va = T.run_matvec(ctx, A, s, "saber", T.VALU, round_to_p=True)
     assert np.array_equal(mf, va)
+    assert np.array_equal(T.run_matvec(ctx, A, s, "saber", T.MFMA32, round_to_p=True), mf)
     raw = O.matvec_batch(3, A, s, method=1, nthreads=16)
     assert np.array_equal(mf, q2p("saber", raw))
 
@@ -259,6 +261,10 @@ def test_argument_errors(ctx):
     assert lib.tmvp_matvec(ctx.handle, 9, T.MFMA, 0, buf.ptr, buf.ptr, buf.ptr, 1, 0, 0, None) == -1
     assert lib.tmvp_matvec(ctx.handle, 1, T.MFMA, 0, buf.ptr, buf.ptr, buf.ptr, 1, 1, T.ACCUMULATE, None) == -1
     assert lib.tmvp_innerprod(ctx.handle, 1, T.MFMA, T.IP_ENC, buf.ptr, buf.ptr, buf.ptr, None, 1, 0, None) == -1
+    both = T.TILE_16 | T.TILE_32  # two tiles at once, or an unknown flag
+    assert lib.tmvp_polymul(ctx.handle, T.MFMA, buf.ptr, buf.ptr, buf.ptr, 1, 13, both, None) == -1
+    assert lib.tmvp_matvec(ctx.handle, 1, T.MFMA, 0, buf.ptr, buf.ptr, buf.ptr, 1, 0, both, None) == -1
+    assert lib.tmvp_innerprod(ctx.handle, 1, T.MFMA, 0, buf.ptr, buf.ptr, buf.ptr, None, 1, 0x40, None) == -1
     assert lib.tmvp_innerprod(ctx.handle, 1, T.MFMA, 5, buf.ptr, buf.ptr, buf.ptr, None, 1, 0, None) == -1
     T.device_synchronize()
 
