@@ -23,7 +23,9 @@ enum Epilogue : int {
     kEpiMod = 0,  // (x [+ old]) mod 2^w
     kEpiQ2P = 1,  // ((x + h) mod 2^eq) >> (eq - ep)
     kEpiEnc = 2,  // ((x + h - ((aux & 1) << (ep-1))) mod 2^ep) >> (ep - et)
-    kEpiDec = 3   // ((x + h - ((aux mod 2^et) << (ep-et))) mod 2^ep) >> (ep - 1)
+    kEpiDec = 3,  // ((x + h - ((aux mod 2^et) << (ep-et))) mod 2^ep) >> (ep - 1)
+    kEpiQ2PAcc = 4 // ((x + old + h) mod 2^eq) >> (eq - ep): the last terms of a row
+                   // whose first terms' sum (mod 2^eq) is already in out
 };
 
 // Everything a product launch needs; passed by value as the kernel argument.
@@ -43,6 +45,10 @@ struct Job {
     int h;               // h1 (q2p, enc) or h2 (dec)
     int eq, ep, et;
     int shape;           // MFMA tile: 16 (16x16x64), 32 (32x32x32), 0 = the library's choice
+    // Elements between consecutive instances of a and s, read only by the
+    // strided form of the MFMA body (the one-launch KEM's partial rows);
+    // every other launch uses the packed layout of its mode.
+    int64_t astride, sstride;
 };
 
 __host__ __device__ inline constexpr int rows_of(int mode, int l)
@@ -63,6 +69,8 @@ __device__ __forceinline__ uint32_t apply_epilogue(const Job &J, uint32_t x, uin
     case kEpiDec:
         return ((x + (uint32_t)J.h - ((aux & ((1u << J.et) - 1u)) << (J.ep - J.et))) & ((1u << J.ep) - 1u)) >>
                (J.ep - 1);
+    case kEpiQ2PAcc:
+        return ((x + old + (uint32_t)J.h) & ((1u << J.eq) - 1u)) >> (J.eq - J.ep);
     default:
         return (J.accumulate ? x + old : x) & ((1u << J.w) - 1u);
     }

@@ -227,6 +227,45 @@ __device__ __forceinline__ void absorb_prefetch(uint32_t (&A)[25], const uint8_t
     A[RW - 1] ^= p ? 0x80000000u : 0u;
 }
 
+// Absorption of a message whose bytes become available in pieces (another
+// wave is still writing the rest): absorb_upto takes every whole rate block of
+// in[done, avail) into A (zeroed by the caller before the first piece),
+// advancing `done`; absorb_final then takes the len - done < RATE bytes left
+// and the padding.  in is 8-byte aligned and done stays a multiple of RATE.
+template <int RATE, int U = 1>
+__device__ __forceinline__ void absorb_upto(uint32_t (&A)[25], const uint8_t *in, int &done, int avail, int p)
+{
+    constexpr int RW = RATE / 8;
+    while (done + RATE <= avail) {
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(in + done) + p;
+        uint32_t m[RW];
+#pragma unroll
+        for (int i = 0; i < RW; i++) m[i] = w[2 * i];
+#pragma unroll
+        for (int i = 0; i < RW; i++) A[i] ^= m[i];
+        f1600<U>(A, p);
+        done += RATE;
+    }
+}
+
+template <int RATE>
+__device__ __forceinline__ void absorb_final(uint32_t (&A)[25], const uint8_t *in, int done, int len, uint8_t ds, int p)
+{
+    constexpr int RW = RATE / 8;
+    const int rest = len - done, full = rest >> 3, rem = rest & 7;
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(in + done) + p;
+    uint64_t last = (uint64_t)ds << (8 * rem);
+    const uint8_t *tail = in + done + 8 * full;
+    for (int b = 0; b < rem; b++) last |= (uint64_t)tail[b] << (8 * b);
+    const uint32_t lh = p ? (uint32_t)(last >> 32) : (uint32_t)last;
+#pragma unroll
+    for (int i = 0; i < RW; i++) {
+        if (i < full) A[i] ^= w[2 * i];
+        else if (i == full) A[i] ^= lh;
+    }
+    A[RW - 1] ^= p ? 0x80000000u : 0u;
+}
+
 // SHA3-256 through absorb_prefetch.
 template <int U = 1>
 __device__ __forceinline__ void sha3_256_prefetch(uint32_t (&out)[4], const uint8_t *in, int len, int p)

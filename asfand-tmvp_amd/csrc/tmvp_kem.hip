@@ -26,6 +26,7 @@
 #include "tmvp_internal.h"
 #include "tmvp_keccak.h"
 #include "tmvp_kem.h"
+#include "tmvp_mfma_body.h"
 
 namespace tmvp {
 
@@ -800,6 +801,426 @@ __global__ __launch_bounds__(64) void dec_fin_kernel(KemParams P, const uint32_t
     if (fail && p == 0) fail[t] = (uint8_t)(mask & 1u);
 }
 
+// ---------------------------------------------------------------- one-launch key generation and encapsulation
+//
+// The MFMA backend runs key generation and encapsulation as ONE launch in
+// which the products and the final hash overlap the matrix sponge.  A
+// workgroup of 16 waves takes 64 instances (the waves go round-robin over
+// the CU's four SIMDs, so each SIMD holds one sponge wave and three MFMA
+// waves):
+//   waves 0, 1   the matrix sponges (GenMatrix) of instances 0-31 and 32-63;
+//   waves 2, 3   the rest of those instances' hashing: the secret, and at the
+//                end the hash over the packed output (H(pk) / H(ct)),
+//                absorbed row by row as the products finish the rows;
+//   waves 4-15   the products on the matrix cores, one output row at a time
+//                (mfma_body as an InnerProd of the row's terms with the
+//                secret), each as soon as its polynomials and the secret
+//                exist.  In encapsulation waves 6 and 7, which share their
+//                SIMDs with the secret waves rather than the matrix sponges,
+//                first hash the public keys.
+// A row's polynomials come out of the matrix stream in order (encapsulation,
+// A s': row i is A[i][0..L); key generation, A^T s: row j is A[0..L)[j], so
+// the rows finish with the stream's last L polynomials), and the hash over
+// the output starts with the first finished row while the stream still runs.
+// Saber, 16384 instances (one workgroup per CU): key generation 224 us and
+// encapsulation 225 us, against 254 and 261 us for the three-launch chains
+// below (same box, alternating runs).  With four MFMA waves per workgroup the
+// products, latency-bound at one MFMA wave per SIMD, became the critical
+// path (each row of 64 instances took 22-48 us), and the launch was slower
+// than the chains.
+//
+// The matrix still goes through memory: a workgroup's share is 64 L^2 x 512 B
+// (288 KB for Saber) against 160 KB of LDS, and in encapsulation most of it
+// exists before the secret that multiplies it.  It is written in the layout
+// the row products read, [row][instance][term][256], and read back from the
+// L2 right after.
+//
+// Progress goes through LDS counters, each raised by one wave or counted up
+// by the MFMA waves (wait_for / publish / fused_leave).  Every wait is on a
+// counter raised by waves that do not themselves wait on the waiter, and
+// every wave raises its counters to their final values even when it has no
+// instances, so every wait ends; a bound on each wait (about 0.2 s) sets the
+// status word instead of hanging if that ever failed.
+constexpr int kFusedInst = 64;
+constexpr int kFusedMfma = 12;                        // MFMA waves per workgroup
+constexpr int kFusedThreads = 64 * (4 + kFusedMfma); // four sponge waves, then the MFMA waves
+constexpr int kSpinLimit = 1 << 22;
+constexpr int kDone = 1 << 20; // a counter's value when its wave has no instances
+
+constexpr int kFusedJobs = 8; // product jobs: at most 2 L (key generation's heads and tails)
+constexpr int kFusedGrab = 2; // instances an MFMA wave takes from a job at a time
+
+struct FusedSync {
+    int mat[2];            // polynomials of A written by matrix wave w (published at the milestones below)
+    int sec[2];            // secret wave w: 1 once the secret (and in encapsulation b and m's bits) is written
+    int hpk[2];            // encapsulation: 1 once H(pk) of group w's instances is in LDS
+    int next[kFusedJobs];  // next unclaimed instance of each product job
+    int done[kFusedJobs];  // MFMA waves that have left each job (all its instances finished at kFusedMfma)
+};
+
+__device__ __forceinline__ void wait_for(const int *f, int v, int *status)
+{
+    int n = 0;
+    while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < v) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++n == kSpinLimit) {
+            *status = 1;
+            break;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ __forceinline__ void publish(int *f, int v)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Stream polynomials written before a product job may start.  Key
+// generation: the heads of every row (terms i < L - 1, A[i][r]) need rows
+// 0..L-2 of the stream, (L - 1) L polynomials; row r's tail (A[L-1][r]) needs
+// (L - 1) L + r + 1.  Encapsulation: row i needs (i + 1) L.
+template <int L> __device__ __forceinline__ constexpr bool milestone(bool keygen, int done)
+{
+    return keygen ? done >= (L - 1) * L : done % L == 0;
+}
+
+// GenMatrix of instance t into A in row-product layout; publishes `flag` at
+// every polynomial count some row waits for.
+template <class SH, int L, bool KEYGEN, int U>
+__device__ __forceinline__ void fused_matrix(const KemParams &P, const uint32_t (&seed)[4], uint16_t *A, int64_t B,
+                                             int64_t t, uint32_t *ring, int p, int *flag)
+{
+    stream_groups<SH::EQ, 3, U>(seed, P.araw / 8, ring, p, [&](int g, const uint32_t(&d)[SH::EQ]) {
+        const int pi = g >> 2, q = g & 3, i = pi / L, j = pi % L; // stream polynomial pi is A[i][j]
+        const int row = KEYGEN ? j : i, term = KEYGEN ? i : j;
+        unpack32<SH::EQ>(d, A + (((int64_t)row * B + t) * L + term) * kN + 64 * q + 32 * p);
+        if (q == 3 && milestone<L>(KEYGEN, pi + 1)) publish(flag, pi + 1);
+    });
+}
+
+// Product job `job`: a product of LT terms for each of the workgroup's
+// instances, one output row (J's pointers are already at its first
+// instance; STRIDED: a sub-range of each instance's terms).  The MFMA waves
+// claim kFusedGrab instances at a time (fused_claim), so waves sharing a SIMD
+// with a busy sponge take fewer; a wave that finds nothing left counts itself
+// out of the job (fused_leave), and the job is finished when all kFusedMfma
+// have.
+template <int LT, int EPI, bool STRIDED>
+__device__ __forceinline__ bool fused_claim(FusedSync &sy, int job, int nloc, Job &J, uint8_t *slices)
+{
+    int i = 0;
+    if ((threadIdx.x & 63) == 0)
+        i = __hip_atomic_fetch_add(&sy.next[job], kFusedGrab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    i = __builtin_amdgcn_readfirstlane(i);
+    if (i >= nloc) return false;
+    J.batch = i + kFusedGrab < nloc ? i + kFusedGrab : nloc;
+    mfma_body<LT, kInner, EPI, 16, STRIDED>(J, i, 1, slices);
+    return true;
+}
+
+__device__ __forceinline__ void fused_leave(FusedSync &sy, int job)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(&sy.done[job], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// J with the pointers of one row job; astride / sstride > 0 select a
+// sub-range of each instance's L terms (the strided form).
+__device__ __forceinline__ Job row_job_of(Job J, const uint16_t *a, const uint16_t *s, uint16_t *out,
+                                          const uint16_t *aux, int w, int epi, int64_t astride = 0, int64_t sstride = 0)
+{
+    J.a = a;
+    J.s = s;
+    J.out = out;
+    J.aux = aux;
+    J.w = w;
+    J.epi = epi;
+    J.astride = astride;
+    J.sstride = sstride;
+    return J;
+}
+
+template <int LT, int EPI, bool STRIDED>
+__device__ __forceinline__ void fused_row(FusedSync &sy, int job, int nloc, Job J, uint8_t *slices)
+{
+    while (fused_claim<LT, EPI, STRIDED>(sy, job, nloc, J, slices)) {
+    }
+    fused_leave(sy, job);
+}
+
+// A key-generation row (a column of A) whose last term arrives late is
+// split: the head (terms 0..L-2) sums into the row's output mod 2^eq as soon
+// as its polynomials exist, and the tail (term L-1) adds the last product and
+// rounds (kEpiQ2PAcc), so only one term's work is left once the stream has
+// written the column.  A tail reads what its head wrote, so it starts only
+// when every wave has left the head.
+template <int L>
+__device__ __forceinline__ Job tail_job_of(const Job &J, const uint16_t *a, const uint16_t *s, uint16_t *out, int eq)
+{
+    return row_job_of(J, a + (L - 1) * kN, s + (L - 1) * kN, out, nullptr, eq, kEpiQ2PAcc, L * kN, L * kN);
+}
+
+template <int L> struct FusedLds {
+    FusedSync sync;
+    uint32_t hpk[kFusedInst * 8];                            // H(pk) per local instance (encapsulation)
+    uint32_t ring[4][32 * kRingDwords];                      // matrix waves 0, 1; secret waves 2, 3 (MU = 6, 10)
+    __attribute__((aligned(16))) uint8_t mfma[kFusedMfma][2 * Slice<L, kInner>::BYTES]; // MFMA waves' slices
+};
+
+template <class SH, int L>
+__global__ __launch_bounds__(kFusedThreads) void kp_fused_kernel(KemParams P, const uint8_t *seeds, uint8_t *pk,
+                                                                 uint8_t *sk, KemWorkspace W, int64_t B, int *status)
+{
+    constexpr int EP = SH::EP;
+    __shared__ FusedLds<L> S;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t base = (int64_t)blockIdx.x * kFusedInst;
+    const int nloc = (int)(B - base < kFusedInst ? B - base : kFusedInst);
+    if (threadIdx.x < sizeof(FusedSync) / 4) reinterpret_cast<int *>(&S.sync)[threadIdx.x] = 0;
+    __syncthreads();
+    const int p = lane & 1;
+    if (wave < 4) {
+        const int grp = wave & 1, li = 32 * grp + (lane >> 1);
+        const int64_t t = base + li;
+        uint32_t *ring = S.ring[wave] + (lane >> 1) * kRingDwords;
+        if (32 * grp >= nloc) {
+            publish(wave < 2 ? &S.sync.mat[grp] : &S.sync.sec[grp], kDone);
+            return;
+        }
+        if (li >= nloc) return; // lane 0 of the wave has an instance and publishes
+        if (wave < 2) {
+            // seed_A' = SHAKE-128(seed_A) to the pk tail and the sk's copy of the pk, then GenMatrix
+            uint32_t m[4], st[25];
+            load_words(m, seeds + t * 96, p);
+            absorb_words(st, m, kShake128Rate, kShakeDS, p);
+            f1600<1>(st, p);
+            const uint32_t sa[4] = {st[0], st[1], st[2], st[3]};
+            store_words(pk + t * P.pk + P.pvc, sa, p);
+            store_words(sk + t * P.sk + P.indcpa_sk + P.pvc, sa, p);
+            fused_matrix<SH, L, true, 1>(P, sa, W.A, B, t, ring, p, &S.sync.mat[grp]);
+            return;
+        }
+        // the secret, packed into the sk, and z
+        uint32_t m[4];
+        load_words(m, seeds + t * 96 + 32, p);
+        uint16_t *s = W.s + t * L * kN;
+        secret_stream<SH::MU, 1>(P, m, s, ring, p);
+        wave_fence_global();
+        if constexpr (SH::MU == 2) pack_half<2, true>(s, sk + t * P.sk, 4 * L, p);
+        else pack_half<SH::EQ, false>(s, sk + t * P.sk, 4 * L, p);
+        load_words(m, seeds + t * 96 + 64, p);
+        store_words(sk + t * P.sk + P.sk - 32, m, p);
+        publish(&S.sync.sec[grp], 1);
+        // b row by row into the pk and the sk's copy, and H(pk) over it as it grows
+        uint8_t *pkt = pk + t * P.pk;
+        uint32_t st[25];
+#pragma unroll
+        for (int i = 0; i < 25; i++) st[i] = 0;
+        int done = 0;
+        for (int r = 0; r < L; r++) {
+            wait_for(&S.sync.done[L + r], kFusedMfma, status);
+            pack_half<EP, false>(W.v1 + ((int64_t)r * B + t) * kN, pkt + r * EP * 32, 4, p,
+                                 sk + t * P.sk + P.indcpa_sk + r * EP * 32);
+            wave_fence_global();
+            absorb_upto<kSha3_256Rate, 2>(st, pkt, done, (r + 1) * EP * 32, p);
+        }
+        // the pk tail (seed_A') was written by the matrix wave before the rows' products began
+        absorb_upto<kSha3_256Rate, 2>(st, pkt, done, P.pk, p);
+        absorb_final<kSha3_256Rate>(st, pkt, done, P.pk, kSha3DS, p);
+        f1600<2>(st, p);
+        const uint32_t h[4] = {st[0], st[1], st[2], st[3]};
+        store_words(sk + t * P.sk + P.sk - 64, h, p);
+        return;
+    }
+    // products: b = round(A^T s) row by row
+    const int k = wave - 4;
+    Job J;
+    memset(&J, 0, sizeof(J));
+    J.status = status;
+    J.batch = nloc;
+    J.eq = P.eq;
+    J.ep = P.ep;
+    J.et = P.et;
+    J.h = P.h1;
+    // Jobs: the heads of rows 0..L-1 (jobs 0..L-1), then the tails (jobs L..2L-1).
+    const uint16_t *s = W.s + base * L * kN;
+    auto a_of = [&](int r) { return W.A + ((int64_t)r * B + base) * L * kN; };
+    auto out_of = [&](int r) { return W.v1 + ((int64_t)r * B + base) * kN; };
+    for (int g = 0; g < 2; g++) wait_for(&S.sync.sec[g], 1, status);
+    for (int g = 0; g < 2; g++) wait_for(&S.sync.mat[g], (L - 1) * L, status);
+    // The heads of every row first, then the tails as their columns are
+    // streamed.  (Taking a tail as soon as it is ready, between claims of the
+    // heads, measured 7 us slower at 16384 instances.)
+    for (int r = 0; r < L; r++)
+        fused_row<L - 1, kEpiMod, true>(
+            S.sync, r, nloc, row_job_of(J, a_of(r), s, out_of(r), nullptr, P.eq, kEpiMod, L * kN, L * kN), S.mfma[k]);
+    for (int r = 0; r < L; r++) {
+        for (int g = 0; g < 2; g++) wait_for(&S.sync.mat[g], (L - 1) * L + r + 1, status);
+        wait_for(&S.sync.done[r], kFusedMfma, status);
+        fused_row<1, kEpiQ2PAcc, true>(S.sync, L + r, nloc, tail_job_of<L>(J, a_of(r), s, out_of(r), P.eq), S.mfma[k]);
+    }
+}
+
+// Encapsulation.  Products: rows 0..L-2 of A s', v' = b . s' (b of the pk,
+// with the message folded in by the encryption epilogue), and row L-1, the
+// last one the matrix stream finishes, as a head and a tail.
+template <class SH, int L, int ET>
+__global__ __launch_bounds__(kFusedThreads) void enc_fused_kernel(KemParams P, const uint8_t *pk, const uint8_t *rnd,
+                                                                  uint8_t *ct, uint8_t *ss, KemWorkspace W, int64_t B,
+                                                                  int *status)
+{
+    constexpr int EP = SH::EP;
+    __shared__ FusedLds<L> S;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t base = (int64_t)blockIdx.x * kFusedInst;
+    const int nloc = (int)(B - base < kFusedInst ? B - base : kFusedInst);
+    if (threadIdx.x < sizeof(FusedSync) / 4) reinterpret_cast<int *>(&S.sync)[threadIdx.x] = 0;
+    __syncthreads();
+    const int p = lane & 1;
+    if (wave < 4) {
+        const int grp = wave & 1, li = 32 * grp + (lane >> 1);
+        const int64_t t = base + li;
+        uint32_t *ring = S.ring[wave] + (lane >> 1) * kRingDwords;
+        if (32 * grp >= nloc) {
+            publish(wave < 2 ? &S.sync.mat[grp] : &S.sync.sec[grp], kDone);
+            return;
+        }
+        if (li >= nloc) return; // lane 0 of the wave has an instance and publishes
+        if (wave < 2) {
+            uint32_t m[4];
+            load_words(m, pk + t * P.pk + P.pvc, p);
+            fused_matrix<SH, L, false, 2>(P, m, W.A, B, t, ring, p, &S.sync.mat[grp]);
+            return;
+        }
+        // m = H(rnd), (K, r) = G(m | H(pk)), the secret from r, b of the pk and m's bits
+        uint32_t r[4], m[4], kb[8], kk[8];
+        load_words(r, rnd + t * 32, p);
+        sha3_256_words<4, 2>(m, r, p);
+        wait_for(&S.sync.hpk[grp], 1, status);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            kb[i] = m[i];
+            kb[4 + i] = S.hpk[li * 8 + 2 * i + p];
+        }
+        sha3_512_words<2>(kk, kb, p);
+        const uint32_t coins[4] = {kk[4], kk[5], kk[6], kk[7]};
+        secret_stream<SH::MU, 2>(P, coins, W.s + t * L * kN, ring, p);
+        message_bits(m, W.bits + t * kN, p);
+        unpack_b<EP>(P, pk + t * P.pk, W.v2 + t * L * kN, p);
+        publish(&S.sync.sec[grp], 1);
+        // the ciphertext row by row, and H(ct) over it as it grows
+        uint8_t *c = ct + t * P.ct;
+        uint32_t st[25];
+#pragma unroll
+        for (int i = 0; i < 25; i++) st[i] = 0;
+        int done = 0;
+        for (int row = 0; row < L; row++) {
+            wait_for(&S.sync.done[row < L - 1 ? row : L], kFusedMfma, status);
+            pack_half<EP, false>(W.v1 + ((int64_t)row * B + t) * kN, c + row * EP * 32, 4, p);
+            wave_fence_global();
+            absorb_upto<kSha3_256Rate, 2>(st, c, done, (row + 1) * EP * 32, p);
+        }
+        wait_for(&S.sync.done[L - 1], kFusedMfma, status); // v' (already finished: every wave left it before row L - 1)
+        pack_half<ET, false>(W.poly + t * kN, c + P.pvc, 4, p);
+        wave_fence_global();
+        absorb_upto<kSha3_256Rate, 2>(st, c, done, P.ct, p);
+        absorb_final<kSha3_256Rate>(st, c, done, P.ct, kSha3DS, p);
+        f1600<2>(st, p);
+        // ss = H(K | H(ct))
+#pragma unroll
+        for (int i = 0; i < 4; i++) kb[4 + i] = st[i];
+#pragma unroll
+        for (int i = 0; i < 4; i++) kb[i] = kk[i];
+        uint32_t sv[4];
+        sha3_256_words<8, 2>(sv, kb, p);
+        store_words(ss + t * 32, sv, p);
+        return;
+    }
+    const int k = wave - 4;
+    if (k == 2 || k == 3) {
+        // H(pk) of group k - 2's instances, for the secret waves.  Waves 6 and
+        // 7 share their SIMDs with the secret waves (waves go round-robin over
+        // the four SIMDs), which wait for this, rather than with the matrix
+        // sponges.
+        const int li = 32 * (k - 2) + (lane >> 1);
+        if (li < nloc) {
+            uint32_t h[4];
+            sha3_256<2>(h, pk + (base + li) * P.pk, P.pk, p);
+#pragma unroll
+            for (int i = 0; i < 4; i++) S.hpk[li * 8 + 2 * i + p] = h[i];
+        }
+        publish(&S.sync.hpk[k - 2], 1);
+    }
+    Job J;
+    memset(&J, 0, sizeof(J));
+    J.status = status;
+    J.batch = nloc;
+    J.eq = P.eq;
+    J.ep = P.ep;
+    J.et = P.et;
+    J.h = P.h1;
+    for (int g = 0; g < 2; g++) wait_for(&S.sync.sec[g], 1, status);
+    // Jobs: rows 0..L-2 (jobs 0..L-2), v' (job L-1), row L-1 (job L).  Row
+    // L - 1 is not split into a head and a tail as in key generation: the
+    // head's products then ran beside the last permutations of the matrix
+    // stream, which slowed down, and encapsulation took 11 us longer.
+    const uint16_t *s = W.s + base * L * kN;
+    auto row_of = [&](int r) {
+        return row_job_of(J, W.A + ((int64_t)r * B + base) * L * kN, s, W.v1 + ((int64_t)r * B + base) * kN, nullptr,
+                          P.eq, kEpiQ2P);
+    };
+    auto streamed = [&](int r, bool wait) {
+        bool ok = true;
+        for (int g = 0; g < 2; g++) {
+            if (wait) wait_for(&S.sync.mat[g], (r + 1) * L, status);
+            else ok = ok && __hip_atomic_load(&S.sync.mat[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >=
+                                (r + 1) * L;
+        }
+        return __builtin_amdgcn_readfirstlane((int)ok) != 0;
+    };
+    for (int r = 0; r < L - 1; r++) {
+        streamed(r, true);
+        fused_row<L, kEpiQ2P, false>(S.sync, r, nloc, row_of(r), S.mfma[k]);
+    }
+    // Row L - 1 is on the critical path: the stream finishes it last, and the
+    // hash over the ciphertext waits for it.  v' is needed only for the
+    // ciphertext's last bytes, so a wave works on v' only while row L - 1 is
+    // not streamed yet, checking before every claim.
+    Job Jv = row_job_of(J, W.v2 + base * L * kN, s, W.poly + base * kN, W.bits + base * kN, P.ep, kEpiEnc);
+    bool v_left = true;
+    while (v_left && !streamed(L - 1, false)) {
+        if (!fused_claim<L, kEpiEnc, false>(S.sync, L - 1, nloc, Jv, S.mfma[k])) {
+            fused_leave(S.sync, L - 1);
+            v_left = false;
+        }
+    }
+    streamed(L - 1, true);
+    fused_row<L, kEpiQ2P, false>(S.sync, L, nloc, row_of(L - 1), S.mfma[k]);
+    if (v_left) fused_row<L, kEpiEnc, false>(S.sync, L - 1, nloc, Jv, S.mfma[k]);
+}
+
+// The parameter set as compile-time values: the sampling shape, L and ET.
+template <class SH, int L_, int ET_> struct SetT {
+    using Sh = SH;
+    static constexpr int L = L_, ET = ET_;
+};
+
+template <class F> static hipError_t by_set(const KemParams &P, F f)
+{
+    auto is = [&](int l, int eq, int mu, int ep, int et) {
+        return P.l == l && P.eq == eq && P.mu == mu && P.ep == ep && P.et == et;
+    };
+    if (is(2, 13, 10, 10, 3)) return f(SetT<Shape<13, 10, 10>, 2, 3>{});
+    if (is(3, 13, 8, 10, 4)) return f(SetT<Shape<13, 8, 10>, 3, 4>{});
+    if (is(4, 13, 6, 10, 6)) return f(SetT<Shape<13, 6, 10>, 4, 6>{});
+    if (is(2, 11, 2, 9, 3)) return f(SetT<Shape<11, 2, 9>, 2, 3>{});
+    if (is(3, 11, 2, 9, 5)) return f(SetT<Shape<11, 2, 9>, 3, 5>{});
+    if (is(4, 11, 2, 10, 3)) return f(SetT<Shape<11, 2, 10>, 4, 3>{});
+    return hipErrorInvalidValue;
+}
+
 // ---------------------------------------------------------------- host chains
 
 static inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
@@ -1019,6 +1440,15 @@ hipError_t kem_keypair(const KemParams &P, const KemWorkspace &W, int backend, c
                        uint8_t *sk, int64_t B, int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
+    if (backend == 0) {
+        // one launch (the one-launch section above)
+        return by_set(P, [&](auto set) {
+            using T = decltype(set);
+            hipLaunchKernelGGL((kp_fused_kernel<typename T::Sh, T::L>), dim3(blocks_for(B, kFusedInst)),
+                               dim3(kFusedThreads), 0, st, P, seeds, pk, sk, W, B, status);
+            return hipGetLastError();
+        });
+    }
     TRY(by_shape(P, [&](auto sh) {
         hipLaunchKernelGGL(kp_front_kernel<decltype(sh)>, dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, seeds, pk, sk,
                            W.A, W.s, B);
@@ -1040,6 +1470,14 @@ hipError_t kem_encaps(const KemParams &P, const KemWorkspace &W, int backend, co
                       uint8_t *ct, uint8_t *ss, int64_t B, int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
+    if (backend == 0) {
+        return by_set(P, [&](auto set) {
+            using T = decltype(set);
+            hipLaunchKernelGGL((enc_fused_kernel<typename T::Sh, T::L, T::ET>), dim3(blocks_for(B, kFusedInst)),
+                               dim3(kFusedThreads), 0, st, P, pk, rnd, ct, ss, W, B, status);
+            return hipGetLastError();
+        });
+    }
     TRY(by_shape(P, [&](auto sh) {
         hipLaunchKernelGGL((enc_front_kernel<decltype(sh), false>), dim3(pair_blocks(2 * B)), dim3(64), 0, st, P, rnd, pk,
                            nullptr, W.buf, W.kr, W.A, W.s, W.v2, W.bits, B, (int64_t)0, nullptr);

@@ -16,7 +16,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesPerEU
     // before it is read from the other one.
     __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsBytes<L, MODE>];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    mfma_body<L, MODE, EPI, SHAPE>(J, (int64_t)blockIdx.x * 4 + wave, (int64_t)gridDim.x * 4, lds);
+    mfma_body<L, MODE, EPI, SHAPE>(J, (int64_t)blockIdx.x * 4 + wave, (int64_t)gridDim.x * 4,
+                                   lds + wave * 2 * Slice<L, MODE>::BYTES);
 }
 
 // The tile shape of a launch whose job leaves it to the library: 16 unless
@@ -130,10 +131,10 @@ static_assert(kMultiLds >= kLdsBytes<3, kMatVec> && kMultiLds >= kLdsBytes<4, kM
               "LDS bound");
 
 template <int K, int SHAPE>
-__device__ __forceinline__ void run_kind(const Job &J, int64_t first, int64_t stride, uint8_t *lds)
+__device__ __forceinline__ void run_kind(const Job &J, int64_t first, int64_t stride, uint8_t *lds, int wave)
 {
     constexpr KindInfo I = kind_info(K);
-    mfma_body<I.l, I.mode, I.epi, SHAPE>(J, first, stride, lds);
+    mfma_body<I.l, I.mode, I.epi, SHAPE>(J, first, stride, lds + wave * 2 * Slice<I.l, I.mode>::BYTES);
 }
 } // namespace
 
@@ -148,12 +149,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SHAPE == 32
     int k = 0;
     while (k + 1 < M.nseg && (int)blockIdx.x >= M.seg[k + 1].block_begin) k++;
     const Segment &G = M.seg[k];
-    const int64_t first =
-        (int64_t)((int)blockIdx.x - G.block_begin) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t first = (int64_t)((int)blockIdx.x - G.block_begin) * 4 + wave;
     const int64_t stride = (int64_t)G.blocks * 4;
     switch (G.kind) {
 #define TMVP_KIND(K)                                                                                                 \
-    case K: run_kind<K, SHAPE>(G.J, first, stride, lds); break;
+    case K: run_kind<K, SHAPE>(G.J, first, stride, lds, wave); break;
         TMVP_KIND(0) TMVP_KIND(1) TMVP_KIND(2) TMVP_KIND(3) TMVP_KIND(4) TMVP_KIND(5) TMVP_KIND(6) TMVP_KIND(7)
         TMVP_KIND(8) TMVP_KIND(9) TMVP_KIND(10) TMVP_KIND(11) TMVP_KIND(12) TMVP_KIND(13) TMVP_KIND(14)
         TMVP_KIND(15) TMVP_KIND(16) TMVP_KIND(17) TMVP_KIND(18) TMVP_KIND(19) TMVP_KIND(20) TMVP_KIND(21)

@@ -120,6 +120,7 @@ __device__ __forceinline__ uint32_t epi2(const Job &J, uint32_t y, uint32_t old,
 {
     const uint32_t two = 0x00010001u;
     if (EPI == kEpiQ2P) return shr16x2(y, J.eq - J.ep) & (((1u << J.ep) - 1u) * two);
+    if (EPI == kEpiQ2PAcc) return shr16x2(add16x2(y, old), J.eq - J.ep) & (((1u << J.ep) - 1u) * two);
     if (EPI == kEpiEnc) {
         const uint32_t m = shl16x2(aux & two, (uint32_t)(J.ep - 1) * two);
         return shr16x2(sub16x2(y, m), J.ep - J.et) & (((1u << J.et) - 1u) * two);
@@ -165,19 +166,21 @@ constexpr int kWavesPerEU = (SHAPE == 32 || MODE == kEnc || rows_of(MODE, L) * t
 template <int L, int MODE> constexpr int kLdsBytes = 4 * 2 * Slice<L, MODE>::BYTES;
 
 // The work of one wave: instances first, first + stride, ... of job J, with
-// the wave's two LDS slices at lds + wave * 2 * Slice::BYTES.  SHAPE is the
+// the wave's own two LDS slices (2 * Slice::BYTES) at slice0.  SHAPE is the
 // MFMA tile: 16 (v_mfma_i32_16x16x64_i8) or 32 (v_mfma_i32_32x32x32_i8).
-template <int L, int MODE, int EPI, int SHAPE = 16>
-__device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t stride, uint8_t *lds)
+// first and stride are wave-uniform (the callers derive them from a
+// readfirstlane'd wave index), so the instance loop's addresses, bounds and
+// branches stay in scalar registers.  STRIDED: instance i's public
+// polynomials start at J.a + i * J.astride and its secret at J.s + i *
+// J.sstride (a sub-range of a longer row: the one-launch KEM's partial rows).
+template <int L, int MODE, int EPI, int SHAPE = 16, bool STRIDED = false>
+__device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t stride, uint8_t *const slice0)
 {
     using S = Slice<L, MODE>;
     constexpr int NR = S::NR, NT = S::NT, NPOLY = S::NPOLY, CJ = S::CJ, ITJ = S::ITJ;
-    // The wave index is uniform; readfirstlane lets the compiler keep the
-    // instance loop (addresses, bounds, branches) in scalar registers.
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63;
     int64_t inst = first;
     if (inst >= J.batch) return; // wave-uniform: each wave owns private LDS slices, no barriers
-    uint8_t *const slice0 = lds + wave * 2 * S::BYTES;
     int buf = 0; // slice holding the term about to be read
     const uint32_t wmask = (1u << J.w) - 1u;
     const uint32_t lo4 = 0x7f7f7f7fu;                 // lo limb mask of four gathered bytes
@@ -221,7 +224,7 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
         int64_t ia = ie; // the instance's matrix (kEnc may read it from a key store)
         if constexpr (MODE == kEnc)
             if (J.akey) ia = J.akey[ie];
-        const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + ia * (NPA * kN));
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + ia * (STRIDED ? J.astride : (int64_t)NPA * kN));
 #pragma unroll
         for (int it = 0; it < ITJ; it++) {
             const int u0 = lane + 64 * it;                             // chunk of this term: row u >> 5, piece u & 31
@@ -236,7 +239,7 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
                 va[j][it] = src[pq * 32 + (u & 31)];
             }
         }
-        vs[j] = reinterpret_cast<const u32x2 *>(J.s + ie * (NT * kN))[j * 64 + lane];
+        vs[j] = reinterpret_cast<const u32x2 *>(J.s + ie * (STRIDED ? J.sstride : (int64_t)NT * kN))[j * 64 + lane];
     };
 #pragma unroll
     for (int j = 0; j < NT; j++) fetch_term(inst, j);
@@ -436,7 +439,8 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
                         continue;
                     }
                     uint32_t old = 0u, auxv = 0u;
-                    if (EPI == kEpiMod && J.accumulate) old = *(const uint32_t *)(obase + r * kN + 16 * h);
+                    if ((EPI == kEpiMod && J.accumulate) || EPI == kEpiQ2PAcc)
+                        old = *(const uint32_t *)(obase + r * kN + 16 * h);
                     if (EPI == kEpiEnc || EPI == kEpiDec) auxv = *(const uint32_t *)(xbase + 16 * h);
                     *(uint32_t *)(obase + r * kN + 16 * h) = epi2<EPI>(J, y, old, auxv);
                 }
@@ -538,7 +542,7 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
                         continue;
                     }
                     uint16_t *o = J.out + inst * (NRO * kN) + r * kN + pos;
-                    const uint32_t old = (EPI == kEpiMod && J.accumulate) ? *(const uint32_t *)o : 0u;
+                    const uint32_t old = ((EPI == kEpiMod && J.accumulate) || EPI == kEpiQ2PAcc) ? *(const uint32_t *)o : 0u;
                     *(uint32_t *)o = epi2<EPI>(J, y, old, auxv);
                 }
             }

@@ -114,6 +114,26 @@ def test_kem_backends_agree_full_batch(ctx):
     assert np.array_equal(ct[sample], want_ct) and np.array_equal(ss[sample], want_ss)
 
 
+@pytest.mark.parametrize("ps", ["saber", "firesaber", "sable1"])
+def test_kem_one_launch_batch_edges(ctx, ps):
+    """The MFMA backend runs key generation and encapsulation in one launch of
+    64 instances per workgroup, split 32 / 32 between two groups of sponge
+    waves: batches that leave a workgroup partly filled, one group empty or a
+    group with a single instance give the oracle's bytes."""
+    rng = np.random.default_rng(21 + PS_ALL.index(ps))
+    for batch in (1, 31, 32, 33, 64, 65, 97, 130):
+        seeds = seeds_for(rng, batch)
+        pk, sk = T.run_kem_keypair(ctx, seeds, ps, T.MFMA)
+        want_pk, want_sk = O.kem_keypair_batch(ps, seeds, nthreads=8)
+        assert np.array_equal(pk, want_pk), (ps, batch)
+        assert np.array_equal(sk, want_sk), (ps, batch)
+        rnd = rng.integers(0, 256, (batch, 32), dtype=np.uint8)
+        ct, ss = T.run_kem_encaps(ctx, pk, rnd, ps, T.MFMA)
+        want_ct, want_ss = O.kem_enc_batch(ps, rnd, want_pk, nthreads=8)
+        assert np.array_equal(ct, want_ct), (ps, batch)
+        assert np.array_equal(ss, want_ss), (ps, batch)
+
+
 def test_kem_workspace_regrowth(ctx):
     """Batches that grow and shrink on one stream reuse and regrow the
     context's workspace without changing results."""
