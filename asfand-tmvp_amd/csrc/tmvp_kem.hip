@@ -329,6 +329,46 @@ __device__ __forceinline__ void pack_half(const uint16_t *S, uint8_t *dst, int n
     }
 }
 
+// pack_half's code of one instance's polynomials, compared with the bytes at
+// dst instead of written there: nonzero when they differ (this lane's half).
+template <int W>
+__device__ __forceinline__ uint32_t pack_compare_half(const uint16_t *S, const uint8_t *dst, int nchunks, int p)
+{
+    constexpr uint32_t mask = (1u << W) - 1u;
+    uint32_t x = 0;
+    for (int c = 0; c < nchunks; c++) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(S + 64 * c + 32 * p);
+        const uint32_t *d = reinterpret_cast<const uint32_t *>(dst + (int64_t)c * W * 8) + p * W;
+        uint4 v[4];
+        uint32_t have[W];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = src[q];
+#pragma unroll
+        for (int i = 0; i < W; i++) have[i] = d[i];
+        uint32_t w[W];
+#pragma unroll
+        for (int i = 0; i < W; i++) w[i] = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t h[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+#pragma unroll
+                for (int e2 = 0; e2 < 2; e2++) {
+                    const int j = 8 * q + 2 * kk + e2;
+                    const uint32_t e = (e2 ? h[kk] >> 16 : h[kk]) & mask;
+                    const int bit = W * j, i = bit >> 5, o = bit & 31;
+                    w[i] |= e << o;
+                    if (o + W > 32) w[i + 1] |= e >> (32 - o);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < W; i++) x |= w[i] ^ have[i];
+    }
+    return x;
+}
+
 // G(m | H(pk)) -> buf = m | H(pk), kr; returns the coins kr[32:64]; the
 // shared first step of encapsulation and of the re-encryption in
 // decapsulation (S/src/cuda_kernel.cu:116-118, :378-379).
@@ -856,6 +896,7 @@ struct FusedSync {
     int hpk[2];            // encapsulation: 1 once H(pk) of group w's instances is in LDS
     int next[kFusedJobs];  // next unclaimed instance of each product job
     int done[kFusedJobs];  // MFMA waves that have left each job (all its instances finished at kFusedMfma)
+    int unp;               // decapsulation: MFMA waves that have decoded their share of the inputs
 };
 
 __device__ __forceinline__ void wait_for(const int *f, int v, int *status)
@@ -1199,6 +1240,178 @@ __global__ __launch_bounds__(kFusedThreads) void enc_fused_kernel(KemParams P, c
     streamed(L - 1, true);
     fused_row<L, kEpiQ2P, false>(S.sync, L, nloc, row_of(L - 1), S.mfma[k]);
     if (v_left) fused_row<L, kEpiEnc, false>(S.sync, L - 1, nloc, Jv, S.mfma[k]);
+}
+
+// Decapsulation in one launch.  The MFMA waves first decode the sk's secret,
+// b' and c_m (waves 6 and 7 then hash the ciphertext), then run the
+// decryption (job 0: m' = dec(b' . s, c_m)), and after the secret waves have
+// turned m' into the re-encryption's secret s'', its products: rows 0..L-2 of
+// A s'' (jobs 1..L-1), v'' = b . s'' with m' folded in (job L) and row L-1
+// (job L + 1), as in encapsulation.  The secret waves compare each finished
+// row with the ciphertext (packing it in registers), and the final hash
+// takes K' or z by the outcome.  The sk's pk supplies the matrix seed and b.
+template <class SH, int L, int ET>
+__global__ __launch_bounds__(kFusedThreads) void dec_fused_kernel(KemParams P, const uint8_t *ct, const uint8_t *sk,
+                                                                  uint8_t *ss, uint8_t *fail, KemWorkspace W, int64_t B,
+                                                                  int *status)
+{
+    constexpr int EP = SH::EP;
+    __shared__ FusedLds<L> S;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int64_t base = (int64_t)blockIdx.x * kFusedInst;
+    const int nloc = (int)(B - base < kFusedInst ? B - base : kFusedInst);
+    if (threadIdx.x < sizeof(FusedSync) / 4) reinterpret_cast<int *>(&S.sync)[threadIdx.x] = 0;
+    __syncthreads();
+    const int p = lane & 1;
+    if (wave < 4) {
+        const int grp = wave & 1, li = 32 * grp + (lane >> 1);
+        const int64_t t = base + li;
+        uint32_t *ring = S.ring[wave] + (lane >> 1) * kRingDwords;
+        if (32 * grp >= nloc) {
+            publish(wave < 2 ? &S.sync.mat[grp] : &S.sync.sec[grp], kDone);
+            return;
+        }
+        if (li >= nloc) return; // lane 0 of the wave has an instance and publishes
+        const uint8_t *skt = sk + t * P.sk;
+        if (wave < 2) {
+            uint32_t m[4];
+            load_words(m, skt + P.indcpa_sk + P.pvc, p);
+            fused_matrix<SH, L, false, 2>(P, m, W.A, B, t, ring, p, &S.sync.mat[grp]);
+            return;
+        }
+        // m' from its bits (bit k of m' is coefficient k of the decryption)
+        wait_for(&S.sync.done[0], kFusedMfma, status);
+        uint32_t m[4];
+        const uint32_t *mb = reinterpret_cast<const uint32_t *>(W.bits + t * kN + 32 * p);
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const uint32_t two = mb[32 * w + j];
+                x |= ((two & 1u) << (2 * j)) | (((two >> 16) & 1u) << (2 * j + 1));
+            }
+            m[w] = x;
+        }
+        // (K', r') = G(m' | H(pk)), the secret s'' from r', b of the pk
+        uint32_t hpk[4], kb[8], kk[8];
+        load_words(hpk, skt + P.sk - 64, p);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            kb[i] = m[i];
+            kb[4 + i] = hpk[i];
+        }
+        sha3_512_words<2>(kk, kb, p);
+        const uint32_t coins[4] = {kk[4], kk[5], kk[6], kk[7]};
+        secret_stream<SH::MU, 2>(P, coins, W.s + t * L * kN, ring, p);
+        unpack_b<EP>(P, skt + P.indcpa_sk, W.v2 + t * L * kN, p);
+        publish(&S.sync.sec[grp], 1);
+        // the re-encryption against the ciphertext, row by row
+        const uint8_t *c = ct + t * P.ct;
+        uint32_t diff = 0;
+        for (int row = 0; row < L; row++) {
+            wait_for(&S.sync.done[row < L - 1 ? row + 1 : L + 1], kFusedMfma, status);
+            diff |= pack_compare_half<EP>(W.v1 + ((int64_t)row * B + t) * kN, c + row * EP * 32, 4, p);
+        }
+        wait_for(&S.sync.done[L], kFusedMfma, status);
+        diff |= pack_compare_half<ET>(W.poly + t * kN, c + P.pvc, 4, p);
+        diff |= partner(diff);
+        // ss = H(K | H(ct)) with K = K' on a match, z otherwise (constant time)
+        wait_for(&S.sync.hpk[grp], 1, status);
+        uint32_t z[4], sv[4];
+        load_words(z, skt + P.sk - 32, p);
+        const uint32_t mask = 0u - (uint32_t)(diff != 0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            kb[i] = kk[i] ^ (mask & (kk[i] ^ z[i]));
+            kb[4 + i] = S.hpk[li * 8 + 2 * i + p];
+        }
+        sha3_256_words<8, 2>(sv, kb, p);
+        store_words(ss + t * 32, sv, p);
+        if (fail && p == 0) fail[t] = (uint8_t)(mask & 1u);
+        return;
+    }
+    const int k = wave - 4;
+    // decode s (sk), b' and c_m (ct), one chunk of 64 coefficients per thread
+    {
+        constexpr int CH = 8 * L + 4; // chunks per instance
+        for (int u = (int)threadIdx.x - 256; u < nloc * CH; u += 64 * kFusedMfma) {
+            const int li = u / CH, c = u - li * CH;
+            const int64_t t = base + li;
+            if (c < 4 * L) {
+                const uint64_t *src = reinterpret_cast<const uint64_t *>(sk + t * P.sk);
+                uint4 *dst = reinterpret_cast<uint4 *>(W.s + t * L * kN + c * 64);
+                if constexpr (SH::MU == 2) unpack_chunk<2, kXfSk2>(src + c * 2, dst);
+                else unpack_chunk<SH::EQ, kXfNone>(src + c * SH::EQ, dst);
+            } else if (c < 8 * L) {
+                unpack_chunk<EP, kXfNone>(reinterpret_cast<const uint64_t *>(ct + t * P.ct) + (c - 4 * L) * EP,
+                                          reinterpret_cast<uint4 *>(W.v2 + t * L * kN + (c - 4 * L) * 64));
+            } else {
+                unpack_chunk<ET, kXfNone>(reinterpret_cast<const uint64_t *>(ct + t * P.ct + P.pvc) + (c - 8 * L) * ET,
+                                          reinterpret_cast<uint4 *>(W.poly + t * kN + (c - 8 * L) * 64));
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_fetch_add(&S.sync.unp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    if (k == 2 || k == 3) {
+        // H(ct) of group k - 2's instances, for the final hash
+        const int li = 32 * (k - 2) + (lane >> 1);
+        if (li < nloc) {
+            uint32_t h[4];
+            sha3_256<2>(h, ct + (base + li) * P.ct, P.ct, p);
+#pragma unroll
+            for (int i = 0; i < 4; i++) S.hpk[li * 8 + 2 * i + p] = h[i];
+        }
+        publish(&S.sync.hpk[k - 2], 1);
+    }
+    Job J;
+    memset(&J, 0, sizeof(J));
+    J.status = status;
+    J.batch = nloc;
+    J.eq = P.eq;
+    J.ep = P.ep;
+    J.et = P.et;
+    // decryption: m' bits to W.bits
+    wait_for(&S.sync.unp, kFusedMfma, status);
+    {
+        Job Jd = row_job_of(J, W.v2 + base * L * kN, W.s + base * L * kN, W.bits + base * kN, W.poly + base * kN, P.ep,
+                            kEpiDec);
+        Jd.h = P.h2;
+        fused_row<L, kEpiDec, false>(S.sync, 0, nloc, Jd, S.mfma[k]);
+    }
+    // the re-encryption
+    J.h = P.h1;
+    for (int g = 0; g < 2; g++) wait_for(&S.sync.sec[g], 1, status);
+    const uint16_t *s = W.s + base * L * kN;
+    auto row_of = [&](int r) {
+        return row_job_of(J, W.A + ((int64_t)r * B + base) * L * kN, s, W.v1 + ((int64_t)r * B + base) * kN, nullptr,
+                          P.eq, kEpiQ2P);
+    };
+    auto streamed = [&](int r, bool wait) {
+        bool ok = true;
+        for (int g = 0; g < 2; g++) {
+            if (wait) wait_for(&S.sync.mat[g], (r + 1) * L, status);
+            else ok = ok && __hip_atomic_load(&S.sync.mat[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >=
+                                (r + 1) * L;
+        }
+        return __builtin_amdgcn_readfirstlane((int)ok) != 0;
+    };
+    for (int r = 0; r < L - 1; r++) {
+        streamed(r, true);
+        fused_row<L, kEpiQ2P, false>(S.sync, r + 1, nloc, row_of(r), S.mfma[k]);
+    }
+    Job Jv = row_job_of(J, W.v2 + base * L * kN, s, W.poly + base * kN, W.bits + base * kN, P.ep, kEpiEnc);
+    bool v_left = true;
+    while (v_left && !streamed(L - 1, false)) {
+        if (!fused_claim<L, kEpiEnc, false>(S.sync, L, nloc, Jv, S.mfma[k])) {
+            fused_leave(S.sync, L);
+            v_left = false;
+        }
+    }
+    streamed(L - 1, true);
+    fused_row<L, kEpiQ2P, false>(S.sync, L + 1, nloc, row_of(L - 1), S.mfma[k]);
+    if (v_left) fused_row<L, kEpiEnc, false>(S.sync, L, nloc, Jv, S.mfma[k]);
 }
 
 // The parameter set as compile-time values: the sampling shape, L and ET.
@@ -1561,6 +1774,14 @@ hipError_t kem_decaps(const KemParams &P, const KemWorkspace &W, int backend, co
                       uint8_t *ss, uint8_t *fail, int64_t B, int *status, hipStream_t st)
 {
     if (B <= 0) return hipSuccess;
+    if (backend == 0) {
+        return by_set(P, [&](auto set) {
+            using T = decltype(set);
+            hipLaunchKernelGGL((dec_fused_kernel<typename T::Sh, T::L, T::ET>), dim3(blocks_for(B, kFusedInst)),
+                               dim3(kFusedThreads), 0, st, P, ct, sk, ss, fail, W, B, status);
+            return hipGetLastError();
+        });
+    }
     return decaps_chain(P, W, backend, ct, sk, ss, fail, B, status, st, false);
 }
 

@@ -116,10 +116,11 @@ def test_kem_backends_agree_full_batch(ctx):
 
 @pytest.mark.parametrize("ps", ["saber", "firesaber", "sable1"])
 def test_kem_one_launch_batch_edges(ctx, ps):
-    """The MFMA backend runs key generation and encapsulation in one launch of
-    64 instances per workgroup, split 32 / 32 between two groups of sponge
-    waves: batches that leave a workgroup partly filled, one group empty or a
-    group with a single instance give the oracle's bytes."""
+    """The MFMA backend runs key generation, encapsulation and decapsulation in
+    one launch of 64 instances per workgroup, split 32 / 32 between two groups
+    of sponge waves: batches that leave a workgroup partly filled, one group
+    empty or a group with a single instance give the oracle's bytes, and a
+    tampered third of the ciphertexts is rejected."""
     rng = np.random.default_rng(21 + PS_ALL.index(ps))
     for batch in (1, 31, 32, 33, 64, 65, 97, 130):
         seeds = seeds_for(rng, batch)
@@ -132,6 +133,13 @@ def test_kem_one_launch_batch_edges(ctx, ps):
         want_ct, want_ss = O.kem_enc_batch(ps, rnd, want_pk, nthreads=8)
         assert np.array_equal(ct, want_ct), (ps, batch)
         assert np.array_equal(ss, want_ss), (ps, batch)
+        bad_rows = list(range(batch - 1, -1, -3))
+        cts = tamper(ct, rng, bad_rows)
+        dec, fail = T.run_kem_decaps(ctx, cts, sk, ps, T.MFMA)
+        assert np.array_equal(dec, O.kem_dec_batch(ps, cts, want_sk, nthreads=8)), (ps, batch)
+        expect_fail = np.zeros(batch, np.uint8)
+        expect_fail[bad_rows] = 1
+        assert np.array_equal(fail, expect_fail), (ps, batch)
 
 
 def test_kem_workspace_regrowth(ctx):
