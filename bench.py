@@ -610,6 +610,8 @@ def main():
                     help="seconds of arrivals per rate, capped at 36000 requests per rate")
     ap.add_argument("--only-kem", action="store_true",
                     help="only the batched KEM leg (no CPU rate): for profiler runs")
+    ap.add_argument("--only-mixed", action="store_true",
+                    help="only the mixed-parameter leg: for profiler runs")
     ap.add_argument("--only-matvec", action="store_true",
                     help="only the headline MFMA MatVec timing (cold, settle, settled): for profiler runs")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -628,6 +630,13 @@ def main():
         ctx = T.Context(0)
         stream = T.Stream()
         print(json.dumps({"kem": kem_bench(ctx, stream, args.kem_batch, args.kem_reps, with_cpu=False)}))
+        stream.destroy()
+        ctx.close()
+        return
+    if args.only_mixed:
+        ctx = T.Context(0)
+        stream = T.Stream()
+        print(json.dumps({"mixed": mixed_bench(ctx, stream)}))
         stream.destroy()
         ctx.close()
         return
