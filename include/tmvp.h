@@ -106,7 +106,10 @@ typedef struct tmvp_ctx tmvp_ctx;
 int tmvp_init(tmvp_ctx **ctx, int device);
 int tmvp_destroy(tmvp_ctx *ctx);
 /* Waits for `stream` and returns TMVP_ERANGE if any MFMA launch since the
- * last call saw an out-of-range secret coefficient (then clears the flag). */
+ * last call saw an out-of-range secret coefficient, or if a one-launch KEM
+ * call's internal wait between its waves ran out (a bound of about 0.2 s that
+ * correct launches never reach; that call's outputs are then not valid).
+ * Clears the flag. */
 int tmvp_ctx_status(tmvp_ctx *ctx, tmvp_stream_t stream);
 const char *tmvp_strerror(int code);
 int tmvp_get_params(int param_set, tmvp_params *out);
@@ -214,9 +217,12 @@ int tmvp_graph_end(tmvp_ctx *ctx, tmvp_stream_t stream, tmvp_graph **graph);
 int tmvp_graph_launch(tmvp_graph *graph, tmvp_stream_t stream);
 int tmvp_graph_destroy(tmvp_graph *graph);
 
-/* ---- batched KEM (a GPU chain of fused kernels: SHAKE-128 GenMatrix and CBD
- * GenSecret in the sponges, MatrixVectorMul / InnerProd with their rounding,
- * codecs, Fujisaki-Okamoto hashes) ----
+/* ---- batched KEM (SHAKE-128 GenMatrix and CBD GenSecret in the sponges,
+ * MatrixVectorMul / InnerProd with their rounding, codecs, Fujisaki-Okamoto
+ * hashes).  With TMVP_BACKEND_MFMA each call is one launch in which the
+ * products and hashes overlap the matrix sponge (64 instances per workgroup
+ * of 16 waves); with TMVP_BACKEND_VALU a chain of kernels.  Both give the
+ * same bytes. ----
  * Byte layouts are the reference's: pk = pack_EP(b) | seed_A,
  * sk = s | pk | H(pk) | z (Saber s as 13-bit words, Sable as 2-bit codes),
  * ct = pack_EP(b') | pack_ET(v').  Buffers are [batch][size] device arrays.
