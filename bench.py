@@ -406,6 +406,39 @@ def mixed_bench(ctx, stream, batch=4096, steps=50):
     return out
 
 
+def graph_chain_bench(ctx, stream, calls=32, steps=50):
+    """Where a graph pays: a chain of short launches.  calls single-instance
+    Saber MatrixVectorMul launches (config 1's shape, each on its own
+    instance) back to back, timed eager and as one replayed graph."""
+    A, s = synthetic_inputs(calls, seed=13)
+    dA, ds_ = T.DeviceBuffer.from_numpy(A), T.DeviceBuffer.from_numpy(s)
+    o1, o2 = T.DeviceBuffer(calls * L * N * 2), T.DeviceBuffer(calls * L * N * 2)
+    ea, es = L * L * N * 2, L * N * 2  # bytes per instance of A and s
+
+    def chain(out):
+        base_o, base_a, base_s = out.ptr.value, dA.ptr.value, ds_.ptr.value
+        for i in range(calls):
+            ctx.matvec(base_o + i * es, base_a + i * ea, base_s + i * es, 1, T.SABER, T.MFMA, round_to_p=True,
+                       stream=stream)
+
+    chain(o1)
+    stream.synchronize()
+    g = ctx.capture(stream, lambda: chain(o2))
+    ms_eager = time_launches(lambda: chain(o1), stream, steps, 5) / steps
+    ms_graph = time_launches(lambda: g.launch(stream), stream, steps, 5) / steps
+    stream.synchronize()
+    same = np.array_equal(o1.download(np.uint16), o2.download(np.uint16))
+    g.destroy()
+    ctx.check(stream)
+    for b in (dA, ds_, o1, o2):
+        b.free()
+    if not same:
+        print("[bench] ERROR: graph replay differs from the eager launches", file=sys.stderr)
+        sys.exit(1)
+    return {"workload": f"{calls} single-instance Saber MatrixVectorMul launches back to back",
+            "eager_ms": ms_eager, "graph_replay_ms": ms_graph, "outputs_equal": bool(same)}
+
+
 def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, seed=9):
     sets = ("saber", "sable2")
     rng = np.random.default_rng(seed)
@@ -638,7 +671,7 @@ def main():
     if args.only_mixed:
         ctx = T.Context(0)
         stream = T.Stream()
-        print(json.dumps({"mixed": mixed_bench(ctx, stream)}))
+        print(json.dumps({"mixed": mixed_bench(ctx, stream), "graph_chain": graph_chain_bench(ctx, stream)}))
         stream.destroy()
         ctx.close()
         return
@@ -760,6 +793,9 @@ def main():
         print(f"[bench] mixed: separate {result['mixed']['separate_launches_ms'] * 1e3:.1f} us, persistent "
               f"{result['mixed']['persistent_autotuned_ms'] * 1e3:.1f} us, graph "
               f"{result['mixed']['graph_replay_ms'] * 1e3:.1f} us", file=sys.stderr, flush=True)
+        result["graph_chain"] = graph_chain_bench(ctx, stream)
+        print(f"[bench] 32 batch-1 launches: eager {result['graph_chain']['eager_ms'] * 1e3:.1f} us, graph "
+              f"{result['graph_chain']['graph_replay_ms'] * 1e3:.1f} us", file=sys.stderr, flush=True)
     if not args.no_kem:
         print(f"[bench] KEM B={args.kem_batch} reps={args.kem_reps} ...", file=sys.stderr, flush=True)
         result["kem"] = kem_bench(ctx, stream, args.kem_batch, args.kem_reps, with_cpu=not args.no_cpu)
