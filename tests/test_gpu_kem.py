@@ -142,6 +142,36 @@ def test_kem_one_launch_batch_edges(ctx, ps):
         assert np.array_equal(fail, expect_fail), (ps, batch)
 
 
+@pytest.mark.parametrize("ps", PS_ALL)
+def test_kem_one_launch_equals_chain(ctx, ps):
+    """Every parameter set at 700 instances (eleven workgroups of the one
+    launch, the last partly filled): the MFMA backend's one-launch kernels and
+    the VALU backend's kernel chains give the same keys, ciphertexts, shared
+    keys and rejections."""
+    rng = np.random.default_rng(40 + PS_ALL.index(ps))
+    batch = 700
+    seeds = seeds_for(rng, batch)
+    pk, sk = T.run_kem_keypair(ctx, seeds, ps, T.MFMA)
+    pk2, sk2 = T.run_kem_keypair(ctx, seeds, ps, T.VALU)
+    assert np.array_equal(pk, pk2) and np.array_equal(sk, sk2)
+    rnd = rng.integers(0, 256, (batch, 32), dtype=np.uint8)
+    ct, ss = T.run_kem_encaps(ctx, pk, rnd, ps, T.MFMA)
+    ct2, ss2 = T.run_kem_encaps(ctx, pk, rnd, ps, T.VALU)
+    assert np.array_equal(ct, ct2) and np.array_equal(ss, ss2)
+    bad_rows = list(range(3, batch, 7))
+    cts = tamper(ct, rng, bad_rows)
+    dec, fail = T.run_kem_decaps(ctx, cts, sk, ps, T.MFMA)
+    dec2, fail2 = T.run_kem_decaps(ctx, cts, sk, ps, T.VALU)
+    assert np.array_equal(dec, dec2) and np.array_equal(fail, fail2)
+    expect_fail = np.zeros(batch, np.uint8)
+    expect_fail[bad_rows] = 1
+    assert np.array_equal(fail, expect_fail)
+    good = np.setdiff1d(np.arange(batch), bad_rows)
+    assert np.array_equal(dec[good], ss[good])
+    sample = rng.choice(batch, 12, replace=False)
+    assert np.array_equal(dec[sample], O.kem_dec_batch(ps, cts[sample], sk[sample], nthreads=8))
+
+
 def test_kem_workspace_regrowth(ctx):
     """Batches that grow and shrink on one stream reuse and regrow the
     context's workspace without changing results."""
