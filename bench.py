@@ -443,7 +443,7 @@ def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, s
     sets = ("saber", "sable2")
     rng = np.random.default_rng(seed)
     srv = T.Server(ctx, max_batch=16384)  # the server's default of four slots
-    pools = {}
+    pools, keys = {}, {}
     try:
         t0 = time.perf_counter()
         for ps in sets:
@@ -457,7 +457,7 @@ def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, s
             ct = T.pinned_array((npool, p["ct_bytes"]))
             ct[:] = T.run_kem_encaps(ctx, pk[ids], rng.integers(0, 256, (npool, 32), dtype=np.uint8), ps, T.MFMA)[0]
             pools[ps] = (ids, rnd, ct)
-            del pk, sk
+            keys[ps] = (pk, sk)
         setup_s = time.perf_counter() - t0
         n_req = int(qps * seconds)
         gaps = rng.exponential(1.0 / qps, n_req)
@@ -470,21 +470,25 @@ def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, s
             off = int(rng.integers(0, 16384 - n + 1))
             ids, rnd, ct = pools[ps]
             plan.append((op, ps, ids[off:off + n], (rnd if op == T.SRV_ENCAPS else ct)[off:off + n]))
-        # outputs land in a pinned ring (results are not read back here; the
-        # serving tests check them)
+        # outputs land in a pinned ring; after the run the last requests whose
+        # region was not reused are checked against the plain batched calls
         ring = T.pinned_array((256 << 20,))
-        ring_off = 0
+        ring_off, ring_gen = 0, 0
+        placed = {}
 
-        def outputs(op, ps, n):
-            nonlocal ring_off
+        def outputs(op, ps, n, tag=None):
+            nonlocal ring_off, ring_gen
             ctb = T.get_params(ps)["ct_bytes"] if op == T.SRV_ENCAPS else 1
             need = n * (ctb + 32)
             if ring_off + need > ring.size:
-                ring_off = 0
+                ring_off, ring_gen = 0, ring_gen + 1
             a = ring[ring_off:ring_off + n * ctb].reshape(n, ctb)
             b = ring[ring_off + n * ctb:ring_off + need].reshape(n, 32)
+            out = (a, b) if op == T.SRV_ENCAPS else (b, a.reshape(n))
+            if tag is not None:
+                placed[tag] = (ring_gen, ring_off, out)
             ring_off += need
-            return (a, b) if op == T.SRV_ENCAPS else (b, a.reshape(n))
+            return out
 
         # warm-up: one small and one full-size request of every (set, operation),
         # through the same input and output paths as the timed requests (the
@@ -509,7 +513,7 @@ def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, s
                 if now >= next_poll:  # collect completions at most every 0.5 ms
                     done += srv.poll(256)
                     next_poll = now + 5e-4
-            srv.submit(op, ps, ids, data, tag=k, out=outputs(op, ps, len(ids)))
+            srv.submit(op, ps, ids, data, tag=k, out=outputs(op, ps, len(ids), tag=k))
         submit_lag_ms = (time.perf_counter() - start - arrivals[-1]) * 1e3
         while len(done) < n_req:
             done += srv.poll(256, timeout_us=100000)
@@ -518,6 +522,23 @@ def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, s
         st["instances_done"] -= 4 * (1 + max_count)
     finally:
         srv.close()
+    # the last (up to) 16 requests whose outputs are still in the ring, against
+    # plain batched calls with the same keys and inputs
+    intact = [k for k in range(n_req - 1, -1, -1)
+              if placed[k][0] == ring_gen or (placed[k][0] == ring_gen - 1 and ring_off <= placed[k][1])][:16]
+    checked_ok = True
+    for k in intact:
+        op, ps, ids, data = plan[k]
+        pk, sk = keys[ps]
+        got = placed[k][2]
+        if op == T.SRV_ENCAPS:
+            want = T.run_kem_encaps(ctx, pk[ids], np.asarray(data), ps, T.MFMA)
+        else:
+            want = T.run_kem_decaps(ctx, np.asarray(data), sk[ids], ps, T.MFMA)
+        checked_ok = checked_ok and all(np.array_equal(g, w) for g, w in zip(got, want))
+    if not checked_ok:
+        print("[bench] ERROR: serving outputs differ from the batched calls", file=sys.stderr)
+        sys.exit(1)
     bad = sum(1 for c in done if c["status"] != 0)
     lat = sorted((c["done_us"] - c["submit_us"]) / 1e3 for c in done)
     queue = sorted((c["start_us"] - c["submit_us"]) / 1e3 for c in done)
@@ -543,6 +564,7 @@ def serving_bench(ctx, qps=2000.0, seconds=3.0, nkeys=100_000, max_count=4096, s
         "slots": st["slots"], "max_concurrent_batches": st["max_concurrent"], "setup_s": setup_s,
         "hbm_budget": st["hbm_budget"], "store_bytes": st["store_bytes"], "arena_bytes": st["arena_bytes"],
         "max_inflight_bytes": st["max_inflight_bytes"],
+        "outputs_checked": len(intact), "outputs_equal_batched": bool(checked_ok),
     }
 
 
