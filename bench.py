@@ -305,7 +305,7 @@ def kem_bench(ctx, stream, batch=16384, reps=100, ps="saber", cpu_seconds=4.0, w
     kf = batch * (perms["keypair"] + perms["encaps"] + perms["decaps"]) / (
         (np.mean(t_kp) + np.mean(t_enc) + np.mean(t_dec)) / 1e3)
     out = {
-        "workload": f"Saber KEM keypair/encaps/decaps, {batch} instances per call",
+        "workload": f"{ps} KEM keypair/encaps/decaps, {batch} instances per call",
         "batch": batch, "reps": reps,
         "keccak_f_per_instance": perms,
         "keccak_f_per_s": kf,
@@ -829,6 +829,16 @@ def main():
                                          "peak_shape": kp["shape"], "peak_by_shape": kp["by_shape"]}
         print(f"[bench] KEM encaps+decaps p99 {result['kem']['encaps_decaps_ms']['p99']:.2f} ms", file=sys.stderr,
               flush=True)
+        # every parameter set at the same batch (20 timed calls each, round trip checked)
+        result["kem_all_sets"] = {}
+        for ps in ("lightsaber", "saber", "firesaber", "sable1", "sable2", "sable3"):
+            r = kem_bench(ctx, stream, args.kem_batch, 20, ps=ps, with_cpu=False)
+            result["kem_all_sets"][ps] = {
+                "keypair_ms_p50": r["keypair_ms"]["p50"], "encaps_ms_p50": r["encaps_ms"]["p50"],
+                "decaps_ms_p50": r["decaps_ms"]["p50"], "keccak_f_per_s": r["keccak_f_per_s"],
+                "encaps_decaps_per_s": r["encaps_decaps_per_s"]}
+        print("[bench] KEM all sets: " + ", ".join(f"{k} {v['encaps_ms_p50'] * 1e3:.0f} us" for k, v in
+                                                   result["kem_all_sets"].items()), file=sys.stderr, flush=True)
     if not args.no_serving:
         result["serving"] = []
         for qps in (float(x) for x in str(args.serve_qps).split(",")):
