@@ -948,7 +948,7 @@ __device__ __forceinline__ void fused_matrix(const KemParams &P, const uint32_t 
 // with a busy sponge take fewer; a wave that finds nothing left counts itself
 // out of the job (fused_leave), and the job is finished when all kFusedMfma
 // have.
-template <int LT, int EPI, bool STRIDED>
+template <int LT, int EPI, bool STRIDED, bool PRE = false>
 __device__ __forceinline__ bool fused_claim(FusedSync &sy, int job, int nloc, Job &J, uint8_t *slices)
 {
     int i = 0;
@@ -957,7 +957,8 @@ __device__ __forceinline__ bool fused_claim(FusedSync &sy, int job, int nloc, Jo
     i = __builtin_amdgcn_readfirstlane(i);
     if (i >= nloc) return false;
     J.batch = i + kFusedGrab < nloc ? i + kFusedGrab : nloc;
-    mfma_body<LT, kInner, EPI, 16, STRIDED>(J, i, 1, slices);
+    static_assert(kFusedGrab <= 2, "PRE loads a claim whole");
+    mfma_body<LT, kInner, EPI, 16, STRIDED, PRE>(J, i, 1, slices);
     return true;
 }
 
@@ -983,10 +984,10 @@ __device__ __forceinline__ Job row_job_of(Job J, const uint16_t *a, const uint16
     return J;
 }
 
-template <int LT, int EPI, bool STRIDED>
+template <int LT, int EPI, bool STRIDED, bool PRE = false>
 __device__ __forceinline__ void fused_row(FusedSync &sy, int job, int nloc, Job J, uint8_t *slices)
 {
-    while (fused_claim<LT, EPI, STRIDED>(sy, job, nloc, J, slices)) {
+    while (fused_claim<LT, EPI, STRIDED, PRE>(sy, job, nloc, J, slices)) {
     }
     fused_leave(sy, job);
 }
@@ -1096,14 +1097,16 @@ __global__ __launch_bounds__(kFusedThreads) void kp_fused_kernel(KemParams P, co
     // all heads, then the tails: FireSaber 377 against 387 us, LightSaber 124
     // against 128, Saber 221 against 222 at 16384 instances.  (Taking a tail
     // as soon as it is ready, checked between the claims of the heads,
-    // measured 7 us slower on Saber.)
+    // measured 7 us slower on Saber.)  The claims load both of their instances up
+    // at the start (PRE): FireSaber 353-354 against 376-377 us, the other
+    // sets within 2 us.
     auto tail = [&](int r) {
         for (int g = 0; g < 2; g++) wait_for(&S.sync.mat[g], (L - 1) * L + r + 1, status);
         wait_for(&S.sync.done[r], kFusedMfma, status);
-        fused_row<1, kEpiQ2PAcc, true>(S.sync, L + r, nloc, tail_job_of<L>(J, a_of(r), s, out_of(r), P.eq), S.mfma[k]);
+        fused_row<1, kEpiQ2PAcc, true, true>(S.sync, L + r, nloc, tail_job_of<L>(J, a_of(r), s, out_of(r), P.eq), S.mfma[k]);
     };
     for (int r = 0; r < L; r++) {
-        fused_row<L - 1, kEpiMod, true>(
+        fused_row<L - 1, kEpiMod, true, true>(
             S.sync, r, nloc, row_job_of(J, a_of(r), s, out_of(r), nullptr, P.eq, kEpiMod, L * kN, L * kN), S.mfma[k]);
         if (r >= 1) tail(r - 1);
     }
@@ -1262,6 +1265,10 @@ __global__ __launch_bounds__(kFusedThreads) void dec_fused_kernel(KemParams P, c
                                                                   int *status)
 {
     constexpr int EP = SH::EP;
+    // Claims load both instances up front for L >= 3 (measured at 16384
+    // instances: Saber 206-208 against 211-212 us; LightSaber 144-145 against
+    // 140-141 us, so its claims keep the one-term-ahead loads).
+    constexpr bool kDecPre = L >= 3;
     __shared__ FusedLds<L> S;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int64_t base = (int64_t)blockIdx.x * kFusedInst;
@@ -1384,7 +1391,7 @@ __global__ __launch_bounds__(kFusedThreads) void dec_fused_kernel(KemParams P, c
         Job Jd = row_job_of(J, W.v2 + base * L * kN, W.s + base * L * kN, W.bits + base * kN, W.poly + base * kN, P.ep,
                             kEpiDec);
         Jd.h = P.h2;
-        fused_row<L, kEpiDec, false>(S.sync, 0, nloc, Jd, S.mfma[k]);
+        fused_row<L, kEpiDec, false, kDecPre>(S.sync, 0, nloc, Jd, S.mfma[k]);
     }
     // the re-encryption
     J.h = P.h1;
@@ -1405,19 +1412,19 @@ __global__ __launch_bounds__(kFusedThreads) void dec_fused_kernel(KemParams P, c
     };
     for (int r = 0; r < L - 1; r++) {
         streamed(r, true);
-        fused_row<L, kEpiQ2P, false>(S.sync, r + 1, nloc, row_of(r), S.mfma[k]);
+        fused_row<L, kEpiQ2P, false, kDecPre>(S.sync, r + 1, nloc, row_of(r), S.mfma[k]);
     }
     Job Jv = row_job_of(J, W.v2 + base * L * kN, s, W.poly + base * kN, W.bits + base * kN, P.ep, kEpiEnc);
     bool v_left = true;
     while (v_left && !streamed(L - 1, false)) {
-        if (!fused_claim<L, kEpiEnc, false>(S.sync, L, nloc, Jv, S.mfma[k])) {
+        if (!fused_claim<L, kEpiEnc, false, kDecPre>(S.sync, L, nloc, Jv, S.mfma[k])) {
             fused_leave(S.sync, L);
             v_left = false;
         }
     }
     streamed(L - 1, true);
-    fused_row<L, kEpiQ2P, false>(S.sync, L + 1, nloc, row_of(L - 1), S.mfma[k]);
-    if (v_left) fused_row<L, kEpiEnc, false>(S.sync, L, nloc, Jv, S.mfma[k]);
+    fused_row<L, kEpiQ2P, false, kDecPre>(S.sync, L + 1, nloc, row_of(L - 1), S.mfma[k]);
+    if (v_left) fused_row<L, kEpiEnc, false, kDecPre>(S.sync, L, nloc, Jv, S.mfma[k]);
 }
 
 // The parameter set as compile-time values: the sampling shape, L and ET.

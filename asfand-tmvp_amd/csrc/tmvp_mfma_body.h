@@ -173,7 +173,11 @@ template <int L, int MODE> constexpr int kLdsBytes = 4 * 2 * Slice<L, MODE>::BYT
 // branches stay in scalar registers.  STRIDED: instance i's public
 // polynomials start at J.a + i * J.astride and its secret at J.s + i *
 // J.sstride (a sub-range of a longer row: the one-launch KEM's partial rows).
-template <int L, int MODE, int EPI, int SHAPE = 16, bool STRIDED = false>
+// PRE2 (16x16x64 only): the wave's range holds at most two instances, and the
+// second one's inputs are loaded with the first's at the start instead of one
+// term ahead (the one-launch KEM's claims, which are too short to hide a load
+// behind one term of work).
+template <int L, int MODE, int EPI, int SHAPE = 16, bool STRIDED = false, bool PRE2 = false>
 __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t stride, uint8_t *const slice0)
 {
     using S = Slice<L, MODE>;
@@ -211,6 +215,9 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     // Registers holding one instance's inputs, term by term.
     u32x4 va[NT][ITJ];
     u32x2 vs[NT];
+    u32x4 vb[PRE2 ? NT : 1][ITJ]; // PRE2: the second instance's inputs
+    u32x2 vsb[PRE2 ? NT : 1];
+    static_assert(!PRE2 || SHAPE == 16, "PRE2 is a 16x16x64 form");
     // Loads of term j of instance i.  Every load is issued on every path (an
     // instance past the end reloads the current one, lanes without a chunk
     // reload another chunk of the same term and ignore it), so the compiler
@@ -219,7 +226,7 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     // Public polynomials per instance in J.a (kEnc: the L x L matrix; its row
     // L is b, read from J.b).
     constexpr int NPA = (MODE == kEnc) ? L * L : NPOLY;
-    auto fetch_term = [&](int64_t i, int j) {
+    auto fetch_to = [&](int64_t i, int j, u32x4 (&da)[ITJ], u32x2 &dsv) {
         const int64_t ie = i < J.batch ? i : inst;
         int64_t ia = ie; // the instance's matrix (kEnc may read it from a key store)
         if constexpr (MODE == kEnc)
@@ -233,16 +240,21 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
             if constexpr (MODE == kEnc) {
                 const u32x4 *row = r < L ? src + (r * L + j) * 32
                                          : reinterpret_cast<const u32x4 *>(J.b + ie * (L * kN)) + j * 32;
-                va[j][it] = row[u & 31];
+                da[it] = row[u & 31];
             } else {
                 const int pq = (MODE == kMatVec) ? r * L + j : (MODE == kMatVecT) ? j * L + r : j;
-                va[j][it] = src[pq * 32 + (u & 31)];
+                da[it] = src[pq * 32 + (u & 31)];
             }
         }
-        vs[j] = reinterpret_cast<const u32x2 *>(J.s + ie * (STRIDED ? J.sstride : (int64_t)NT * kN))[j * 64 + lane];
+        dsv = reinterpret_cast<const u32x2 *>(J.s + ie * (STRIDED ? J.sstride : (int64_t)NT * kN))[j * 64 + lane];
     };
+    auto fetch_term = [&](int64_t i, int j) { fetch_to(i, j, va[j], vs[j]); };
 #pragma unroll
     for (int j = 0; j < NT; j++) fetch_term(inst, j);
+    if constexpr (PRE2) {
+#pragma unroll
+        for (int j = 0; j < NT; j++) fetch_to(inst + stride, j, vb[j], vsb[j]);
+    }
 
     uint32_t bad = 0;
     // Stage term j from va[j] / vs[j] into slice `into`.
@@ -323,7 +335,7 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     // and is issued between the MFMAs of the current term.
     stage(0, 0);
     pin_after_stage();
-    fetch_term(inst + stride, 0);
+    if constexpr (!PRE2) fetch_term(inst + stride, 0);
     wave_fence();
     // Accumulators start at the rounding constant h in the lo-limb lanes, so
     // the epilogue never adds it.
@@ -368,11 +380,19 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
                 if (j + 1 < NT) {
                     stage(j + 1, buf ^ 1);
                     pin_after_stage();
-                    fetch_term(next, j + 1);
+                    if constexpr (!PRE2) fetch_term(next, j + 1);
                 } else if (next < J.batch) {
+                    if constexpr (PRE2) {
+#pragma unroll
+                        for (int jj = 0; jj < NT; jj++) {
+#pragma unroll
+                            for (int it = 0; it < ITJ; it++) va[jj][it] = vb[jj][it];
+                            vs[jj] = vsb[jj];
+                        }
+                    }
                     stage(0, buf ^ 1); // va[0] / vs[0] already hold the next instance's term 0
                     pin_after_stage();
-                    fetch_term(next + stride, 0);
+                    if constexpr (!PRE2) fetch_term(next + stride, 0);
                 }
 
                 // K-chunk outermost: 2*NR independent accumulators between dependent MFMAs.
