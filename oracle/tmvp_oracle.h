@@ -8,6 +8,14 @@
  * cpu_baseline leg of bench.py may load it.
  *
  * Every function cites the reference file:line whose behaviour it restates.
+ *
+ * Parity unpinned: or_indcpa_keypair's seed_A = SHAKE-128(seed_A) step and
+ * the KEM's ciphertext / shared-key bytes.  The reference has no key
+ * generation and holds no KEM known-answer vectors, so those are checked
+ * only through their pinned parts (GenMatrix, GenSecret, the transposed
+ * product and its rounding against the reference's public keys, the exact
+ * decoding window of the encryption rounding) and by round trips.
+ *
  * Paths are relative to the reference root:
  *   S/  = TMVP-Saber/SABER-GPU/
  *   SB/ = TMVP_SABLE/
