@@ -143,6 +143,33 @@ def test_kem_one_launch_batch_edges(ctx, ps):
 
 
 @pytest.mark.parametrize("ps", PS_ALL)
+def test_kem_one_launch_random_batches(ctx, ps):
+    """Seeded random batch sizes up to 2500 (partly filled workgroups, claims
+    of one or two instances at a product job's end): every operation of the
+    one launch gives the oracle's bytes, a repeated key generation gives the
+    same bytes, and every second ciphertext, tampered, is rejected."""
+    rng = np.random.default_rng(60 + PS_ALL.index(ps))
+    for batch in [int(x) for x in rng.integers(1, 2500, 4)]:
+        seeds = seeds_for(rng, batch)
+        pk, sk = T.run_kem_keypair(ctx, seeds, ps, T.MFMA)
+        again = T.run_kem_keypair(ctx, seeds, ps, T.MFMA)
+        want_pk, want_sk = O.kem_keypair_batch(ps, seeds, nthreads=8)
+        assert np.array_equal(pk, want_pk) and np.array_equal(sk, want_sk), (ps, batch)
+        assert np.array_equal(again[0], pk) and np.array_equal(again[1], sk), (ps, batch)
+        rnd = rng.integers(0, 256, (batch, 32), dtype=np.uint8)
+        ct, ss = T.run_kem_encaps(ctx, pk, rnd, ps, T.MFMA)
+        want_ct, want_ss = O.kem_enc_batch(ps, rnd, want_pk, nthreads=8)
+        assert np.array_equal(ct, want_ct) and np.array_equal(ss, want_ss), (ps, batch)
+        bad_rows = list(range(0, batch, 2))
+        cts = tamper(ct, rng, bad_rows)
+        dec, fail = T.run_kem_decaps(ctx, cts, sk, ps, T.MFMA)
+        assert np.array_equal(dec, O.kem_dec_batch(ps, cts, want_sk, nthreads=8)), (ps, batch)
+        expect_fail = np.zeros(batch, np.uint8)
+        expect_fail[bad_rows] = 1
+        assert np.array_equal(fail, expect_fail), (ps, batch)
+
+
+@pytest.mark.parametrize("ps", PS_ALL)
 def test_kem_one_launch_equals_chain(ctx, ps):
     """Every parameter set at 700 instances (eleven workgroups of the one
     launch, the last partly filled): the MFMA backend's one-launch kernels and
