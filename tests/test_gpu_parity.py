@@ -106,6 +106,26 @@ def test_matvec_vs_oracle(ctx, backend, ps, transpose):
     assert np.array_equal(got, q2p(ps, raw))
 
 
+@pytest.mark.parametrize("backend", [T.MFMA, T.MFMA32])
+@pytest.mark.parametrize("ps", PS_ALL)
+def test_matvec_random_large_batches_vs_oracle(ctx, backend, ps):
+    """Seeded random batch sizes past one round of the persistent grid (each
+    wave walks several instances, the last round partly filled), both
+    transposes, raw and rounded, against the oracle."""
+    p = O.params(ps)
+    l = p["l"]
+    rng = np.random.default_rng(300 + PS_ALL.index(ps))
+    for batch in [int(x) for x in rng.integers(4097, 20000, 2)]:
+        A = public(rng, (batch, l * l * N), p["eq"])
+        s = secret(rng, (batch, l * N), ps)
+        for transpose in (False, True):
+            raw = O.matvec_batch(l, A, s, transpose=transpose, method=1, nthreads=8).reshape(batch, l, N)
+            got = T.run_matvec(ctx, A, s, ps, backend, transpose=transpose)
+            assert np.array_equal(got, raw & ((1 << p["eq"]) - 1)), (batch, transpose)
+            got = T.run_matvec(ctx, A, s, ps, backend, transpose=transpose, round_to_p=True)
+            assert np.array_equal(got, q2p(ps, raw)), (batch, transpose)
+
+
 @pytest.mark.parametrize("backend", BACKENDS)
 def test_matvec_accumulate(ctx, backend):
     rng = np.random.default_rng(11)
