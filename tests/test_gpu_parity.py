@@ -162,6 +162,36 @@ def test_innerprod_vs_oracle(ctx, backend, ps):
     assert np.array_equal(got, want.astype(np.uint16))
 
 
+@pytest.mark.parametrize("ps", PS_ALL)
+def test_innerprod_and_polymul_random_large_batches(ctx, ps):
+    """InnerProd with each epilogue, and polymul, at seeded random batch sizes
+    past one round of the persistent grid (MFMA, both tiles) against the
+    oracle."""
+    p = O.params(ps)
+    l, ep, et = p["l"], p["ep"], p["et"]
+    rng = np.random.default_rng(400 + PS_ALL.index(ps))
+    batch = int(rng.integers(8193, 30000))
+    b = public(rng, (batch, l * N), ep)
+    s = secret(rng, (batch, l * N), ps)
+    raw = O.innerprod_batch(l, b, s, nthreads=8).astype(np.int64)
+    mbits = rng.integers(0, 2, (batch, N), dtype=np.uint16)
+    cm = rng.integers(0, 1 << et, (batch, N), dtype=np.uint16)
+    want_enc = ((raw + p["h1"] - (mbits.astype(np.int64) << (ep - 1))) % (1 << ep)) >> (ep - et)
+    want_dec = ((raw + p["h2"] - (cm.astype(np.int64) << (ep - et))) % (1 << ep)) >> (ep - 1)
+    for backend in (T.MFMA, T.MFMA32):
+        got = T.run_innerprod(ctx, b, s, ps, backend, T.IP_RAW)
+        assert np.array_equal(got, (raw % (1 << ep)).astype(np.uint16)), (backend, batch)
+        got = T.run_innerprod(ctx, b, s, ps, backend, T.IP_ENC, aux=mbits)
+        assert np.array_equal(got, want_enc.astype(np.uint16)), (backend, batch)
+        got = T.run_innerprod(ctx, b, s, ps, backend, T.IP_DEC, aux=cm)
+        assert np.array_equal(got, want_dec.astype(np.uint16)), (backend, batch)
+    a = public(rng, (batch, N), p["eq"])
+    s1 = s.reshape(batch, l, N)[:, 0]
+    want = O.polymul_batch(a, s1, method=1, nthreads=8) & ((1 << p["eq"]) - 1)
+    for backend in (T.MFMA, T.MFMA32):
+        assert np.array_equal(T.run_polymul(ctx, a, s1, p["eq"], backend), want), (backend, batch)
+
+
 @pytest.mark.parametrize("backend", BACKENDS)
 @pytest.mark.parametrize("ps", PS_ALL)
 def test_enc_dec_epilogues_decoding_window_on_gpu(ctx, backend, ps):
