@@ -189,8 +189,8 @@ def time_launches(fn, stream, steps, warmup):
     return ms
 
 
-# The GPU box gives one GPU's process a 16-core share of its host, whatever
-# the machine's core count: the thread pools use at most that many.
+# A one-GPU process is given a 16-core share of its host, whatever the
+# machine's core count: the thread pools use at most that many.
 CPU_SHARE = 16
 
 
