@@ -948,7 +948,7 @@ __device__ __forceinline__ void fused_matrix(const KemParams &P, const uint32_t 
 // with a busy sponge take fewer; a wave that finds nothing left counts itself
 // out of the job (fused_leave), and the job is finished when all kFusedMfma
 // have.
-template <int LT, int EPI, bool STRIDED, bool PRE = false>
+template <int LT, int EPI, bool STRIDED, bool PRE = false, bool QUAD = true>
 __device__ __forceinline__ bool fused_claim(FusedSync &sy, int job, int nloc, Job &J, uint8_t *slices)
 {
     int i = 0;
@@ -958,7 +958,7 @@ __device__ __forceinline__ bool fused_claim(FusedSync &sy, int job, int nloc, Jo
     if (i >= nloc) return false;
     J.batch = i + kFusedGrab < nloc ? i + kFusedGrab : nloc;
     static_assert(kFusedGrab <= 2, "PRE loads a claim whole");
-    mfma_body<LT, kInner, EPI, 16, STRIDED, PRE>(J, i, 1, slices);
+    mfma_body<LT, kInner, EPI, 16, STRIDED, PRE, QUAD>(J, i, 1, slices);
     return true;
 }
 
@@ -984,10 +984,10 @@ __device__ __forceinline__ Job row_job_of(Job J, const uint16_t *a, const uint16
     return J;
 }
 
-template <int LT, int EPI, bool STRIDED, bool PRE = false>
+template <int LT, int EPI, bool STRIDED, bool PRE = false, bool QUAD = true>
 __device__ __forceinline__ void fused_row(FusedSync &sy, int job, int nloc, Job J, uint8_t *slices)
 {
-    while (fused_claim<LT, EPI, STRIDED, PRE>(sy, job, nloc, J, slices)) {
+    while (fused_claim<LT, EPI, STRIDED, PRE, QUAD>(sy, job, nloc, J, slices)) {
     }
     fused_leave(sy, job);
 }
@@ -1099,14 +1099,18 @@ __global__ __launch_bounds__(kFusedThreads) void kp_fused_kernel(KemParams P, co
     // as soon as it is ready, checked between the claims of the heads,
     // measured 7 us slower on Saber.)  The claims load both of their instances up
     // at the start (PRE): FireSaber 353-354 against 376-377 us, the other
-    // sets within 2 us.
+    // sets within 2 us.  They keep eight-coefficient staging on half the
+    // lanes (QUAD off): with four coefficients per lane key generation took
+    // 229 against 221 us (Saber) and 372 against 354 us (FireSaber), while
+    // decapsulation, which keeps it, gained 6 and 28 us.
     auto tail = [&](int r) {
         for (int g = 0; g < 2; g++) wait_for(&S.sync.mat[g], (L - 1) * L + r + 1, status);
         wait_for(&S.sync.done[r], kFusedMfma, status);
-        fused_row<1, kEpiQ2PAcc, true, true>(S.sync, L + r, nloc, tail_job_of<L>(J, a_of(r), s, out_of(r), P.eq), S.mfma[k]);
+        fused_row<1, kEpiQ2PAcc, true, true, false>(S.sync, L + r, nloc, tail_job_of<L>(J, a_of(r), s, out_of(r), P.eq),
+                                                    S.mfma[k]);
     };
     for (int r = 0; r < L; r++) {
-        fused_row<L - 1, kEpiMod, true, true>(
+        fused_row<L - 1, kEpiMod, true, true, false>(
             S.sync, r, nloc, row_job_of(J, a_of(r), s, out_of(r), nullptr, P.eq, kEpiMod, L * kN, L * kN), S.mfma[k]);
         if (r >= 1) tail(r - 1);
     }
@@ -1212,6 +1216,9 @@ __global__ __launch_bounds__(kFusedThreads) void enc_fused_kernel(KemParams P, c
     J.et = P.et;
     J.h = P.h1;
     for (int g = 0; g < 2; g++) wait_for(&S.sync.sec[g], 1, status);
+    // Claims keep one-term-ahead loads (PRE off: Saber 236 against 224 us)
+    // and eight-coefficient staging (QUAD off: Saber 233 against 223 us,
+    // FireSaber 360 against 330 us).
     // Jobs: rows 0..L-2 (jobs 0..L-2), v' (job L-1), row L-1 (job L).  Row
     // L - 1 is not split into a head and a tail as in key generation: the
     // head's products then ran beside the last permutations of the matrix
@@ -1232,7 +1239,7 @@ __global__ __launch_bounds__(kFusedThreads) void enc_fused_kernel(KemParams P, c
     };
     for (int r = 0; r < L - 1; r++) {
         streamed(r, true);
-        fused_row<L, kEpiQ2P, false>(S.sync, r, nloc, row_of(r), S.mfma[k]);
+        fused_row<L, kEpiQ2P, false, false, false>(S.sync, r, nloc, row_of(r), S.mfma[k]);
     }
     // Row L - 1 is on the critical path: the stream finishes it last, and the
     // hash over the ciphertext waits for it.  v' is needed only for the
@@ -1241,14 +1248,14 @@ __global__ __launch_bounds__(kFusedThreads) void enc_fused_kernel(KemParams P, c
     Job Jv = row_job_of(J, W.v2 + base * L * kN, s, W.poly + base * kN, W.bits + base * kN, P.ep, kEpiEnc);
     bool v_left = true;
     while (v_left && !streamed(L - 1, false)) {
-        if (!fused_claim<L, kEpiEnc, false>(S.sync, L - 1, nloc, Jv, S.mfma[k])) {
+        if (!fused_claim<L, kEpiEnc, false, false, false>(S.sync, L - 1, nloc, Jv, S.mfma[k])) {
             fused_leave(S.sync, L - 1);
             v_left = false;
         }
     }
     streamed(L - 1, true);
-    fused_row<L, kEpiQ2P, false>(S.sync, L, nloc, row_of(L - 1), S.mfma[k]);
-    if (v_left) fused_row<L, kEpiEnc, false>(S.sync, L - 1, nloc, Jv, S.mfma[k]);
+    fused_row<L, kEpiQ2P, false, false, false>(S.sync, L, nloc, row_of(L - 1), S.mfma[k]);
+    if (v_left) fused_row<L, kEpiEnc, false, false, false>(S.sync, L - 1, nloc, Jv, S.mfma[k]);
 }
 
 // Decapsulation in one launch.  The MFMA waves first decode the sk's secret,

@@ -63,6 +63,8 @@
 // (single job and mixed persistent) are in tmvp_mfma.hip.
 #pragma once
 
+#include <type_traits>
+
 #include "tmvp_internal.h"
 
 namespace tmvp {
@@ -176,8 +178,9 @@ template <int L, int MODE> constexpr int kLdsBytes = 4 * 2 * Slice<L, MODE>::BYT
 // PRE2 (16x16x64 only): the wave's range holds at most two instances, and the
 // second one's inputs are loaded with the first's at the start instead of one
 // term ahead (the one-launch KEM's claims, which are too short to hide a load
-// behind one term of work).
-template <int L, int MODE, int EPI, int SHAPE = 16, bool STRIDED = false, bool PRE2 = false>
+// behind one term of work).  QUAD (one public row only): each lane stages
+// four coefficients of a term instead of half the lanes staging eight.
+template <int L, int MODE, int EPI, int SHAPE = 16, bool STRIDED = false, bool PRE2 = false, bool QUAD = true>
 __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t stride, uint8_t *const slice0)
 {
     using S = Slice<L, MODE>;
@@ -213,9 +216,13 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
                                   : rr * 2 * kEPoly + limb * kEPoly + 16 * kh + (R + 8) * kEBlock;
 
     // Registers holding one instance's inputs, term by term.
-    u32x4 va[NT][ITJ];
+    // One public row (InnerProd, polymul): each lane loads and stages four
+    // coefficients of the term, so all 64 lanes share the staging work.
+    constexpr bool HALF = QUAD && CJ == 32;
+    using PubT = std::conditional_t<HALF, u32x2, u32x4>;
+    PubT va[NT][ITJ];
     u32x2 vs[NT];
-    u32x4 vb[PRE2 ? NT : 1][ITJ]; // PRE2: the second instance's inputs
+    PubT vb[PRE2 ? NT : 1][ITJ]; // PRE2: the second instance's inputs
     u32x2 vsb[PRE2 ? NT : 1];
     static_assert(!PRE2 || SHAPE == 16, "PRE2 is a 16x16x64 form");
     // Loads of term j of instance i.  Every load is issued on every path (an
@@ -226,12 +233,16 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     // Public polynomials per instance in J.a (kEnc: the L x L matrix; its row
     // L is b, read from J.b).
     constexpr int NPA = (MODE == kEnc) ? L * L : NPOLY;
-    auto fetch_to = [&](int64_t i, int j, u32x4 (&da)[ITJ], u32x2 &dsv) {
+    auto fetch_to = [&](int64_t i, int j, PubT (&da)[ITJ], u32x2 &dsv) {
         const int64_t ie = i < J.batch ? i : inst;
         int64_t ia = ie; // the instance's matrix (kEnc may read it from a key store)
         if constexpr (MODE == kEnc)
             if (J.akey) ia = J.akey[ie];
         const u32x4 *src = reinterpret_cast<const u32x4 *>(J.a + ia * (STRIDED ? J.astride : (int64_t)NPA * kN));
+        if constexpr (HALF) {
+            // the term's row is polynomial j of the instance (InnerProd) or its only one (polymul)
+            da[0] = reinterpret_cast<const u32x2 *>(src)[j * 64 + lane];
+        } else {
 #pragma unroll
         for (int it = 0; it < ITJ; it++) {
             const int u0 = lane + 64 * it;                             // chunk of this term: row u >> 5, piece u & 31
@@ -245,6 +256,7 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
                 const int pq = (MODE == kMatVec) ? r * L + j : (MODE == kMatVecT) ? j * L + r : j;
                 da[it] = src[pq * 32 + (u & 31)];
             }
+        }
         }
         dsv = reinterpret_cast<const u32x2 *>(J.s + ie * (STRIDED ? J.sstride : (int64_t)NT * kN))[j * 64 + lane];
     };
@@ -262,27 +274,42 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
         uint8_t *const E = slice0 + into * S::BYTES;
         uint8_t *const X = E + S::E_BYTES;
         // ---- public polynomials of term j -> byte-reversed limb blocks E[r][limb][e]
-#pragma unroll
-        for (int it = 0; it < ITJ; it++) {
-            // Lanes past the last chunk skip the staging.  (A branch-free form,
-            // where they rewrite a copy of another lane's chunk, lets the
-            // scheduler mix the staging, which waits for the prefetched loads,
-            // into the MFMAs, and measured 4 us slower at B = 65536.)
-            const int u = lane + 64 * it;
-            if (CJ % 64 != 0 && u >= CJ) continue;
-            const int r = u >> 5;
-            const u32x4 v = va[j][it];
-            // hi limb = coefficient >> 7, per 16-bit half
-            const uint32_t s0 = shr16x2(v.x, 7), s1 = shr16x2(v.y, 7), s2 = shr16x2(v.z, 7), s3 = shr16x2(v.w, 7);
-            // gather the low byte of every coefficient, coefficient u of the chunk to byte 7 - u
-            const uint2 plo = make_uint2(perm(v.w, v.z, 0x00020406u) & lo4, perm(v.y, v.x, 0x00020406u) & lo4);
-            const uint2 phi = make_uint2(perm(s3, s2, 0x00020406u) & hi4, perm(s1, s0, 0x00020406u) & hi4);
-            uint8_t *slot = E + r * 2 * kEPoly + bpos;
-            *(uint2 *)(slot + (blk + 8) * kEBlock) = plo;
-            *(uint2 *)(slot + kEPoly + (blk + 8) * kEBlock) = phi;
-            // negated copy; for blk = 0 it lands in block 0, which no MFMA reads
-            *(uint2 *)(slot + blk * kEBlock) = make_uint2(neg_small(plo.x), neg_small(plo.y));
-            *(uint2 *)(slot + kEPoly + blk * kEBlock) = make_uint2(neg_small(phi.x), neg_small(phi.y));
+        if constexpr (HALF) {
+            // Lane l holds coefficients 4l..4l+3 (block l >> 3, piece l & 7);
+            // coefficient c of a block goes to byte 31 - c, as below.
+            const u32x2 v = va[j][0];
+            const uint32_t s0 = shr16x2(v.x, 7), s1 = shr16x2(v.y, 7);
+            const uint32_t plo = perm(v.y, v.x, 0x00020406u) & lo4;
+            const uint32_t phi = perm(s1, s0, 0x00020406u) & hi4;
+            uint8_t *slot = E + 28 - 4 * (lane & 7);
+            const int b4 = lane >> 3;
+            *(uint32_t *)(slot + (b4 + 8) * kEBlock) = plo;
+            *(uint32_t *)(slot + kEPoly + (b4 + 8) * kEBlock) = phi;
+            *(uint32_t *)(slot + b4 * kEBlock) = neg_small(plo);
+            *(uint32_t *)(slot + kEPoly + b4 * kEBlock) = neg_small(phi);
+        } else {
+    #pragma unroll
+            for (int it = 0; it < ITJ; it++) {
+                // Lanes past the last chunk skip the staging.  (A branch-free form,
+                // where they rewrite a copy of another lane's chunk, lets the
+                // scheduler mix the staging, which waits for the prefetched loads,
+                // into the MFMAs, and measured 4 us slower at B = 65536.)
+                const int u = lane + 64 * it;
+                if (CJ % 64 != 0 && u >= CJ) continue;
+                const int r = u >> 5;
+                const u32x4 v = va[j][it];
+                // hi limb = coefficient >> 7, per 16-bit half
+                const uint32_t s0 = shr16x2(v.x, 7), s1 = shr16x2(v.y, 7), s2 = shr16x2(v.z, 7), s3 = shr16x2(v.w, 7);
+                // gather the low byte of every coefficient, coefficient u of the chunk to byte 7 - u
+                const uint2 plo = make_uint2(perm(v.w, v.z, 0x00020406u) & lo4, perm(v.y, v.x, 0x00020406u) & lo4);
+                const uint2 phi = make_uint2(perm(s3, s2, 0x00020406u) & hi4, perm(s1, s0, 0x00020406u) & hi4);
+                uint8_t *slot = E + r * 2 * kEPoly + bpos;
+                *(uint2 *)(slot + (blk + 8) * kEBlock) = plo;
+                *(uint2 *)(slot + kEPoly + (blk + 8) * kEBlock) = phi;
+                // negated copy; for blk = 0 it lands in block 0, which no MFMA reads
+                *(uint2 *)(slot + blk * kEBlock) = make_uint2(neg_small(plo.x), neg_small(plo.y));
+                *(uint2 *)(slot + kEPoly + blk * kEBlock) = make_uint2(neg_small(phi.x), neg_small(phi.y));
+            }
         }
         // ---- secret s_j -> int8 band X[x] = ts(x - 32), stored 4x shifted: copy c
         //      holds X[x + c] at byte x.  For w >= 8 the centred int8 is the low byte.
