@@ -218,11 +218,17 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     // Registers holding one instance's inputs, term by term.
     // One public row (InnerProd, polymul): each lane loads and stages four
     // coefficients of the term, so all 64 lanes share the staging work.
+    // Three rows (the L = 3 MatrixVectorMul forms, 96 eight-coefficient
+    // chunks, which leave half the lanes idle in the second pass): three
+    // passes of four coefficients per lane (Saber, 65536 instances: 103.9-
+    // 106.7 against 105.4-107.5 us in alternating runs, 120 VGPRs against 128).
     constexpr bool HALF = QUAD && CJ == 32;
-    using PubT = std::conditional_t<HALF, u32x2, u32x4>;
-    PubT va[NT][ITJ];
+    constexpr bool Q3 = QUAD && CJ == 96 && (MODE == kMatVec || MODE == kMatVecT);
+    constexpr int ITQ = Q3 ? NR : ITJ;
+    using PubT = std::conditional_t<HALF || Q3, u32x2, u32x4>;
+    PubT va[NT][ITQ];
     u32x2 vs[NT];
-    PubT vb[PRE2 ? NT : 1][ITJ]; // PRE2: the second instance's inputs
+    PubT vb[PRE2 ? NT : 1][ITQ]; // PRE2: the second instance's inputs
     u32x2 vsb[PRE2 ? NT : 1];
     static_assert(!PRE2 || SHAPE == 16, "PRE2 is a 16x16x64 form");
     // Loads of term j of instance i.  Every load is issued on every path (an
@@ -233,7 +239,7 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
     // Public polynomials per instance in J.a (kEnc: the L x L matrix; its row
     // L is b, read from J.b).
     constexpr int NPA = (MODE == kEnc) ? L * L : NPOLY;
-    auto fetch_to = [&](int64_t i, int j, PubT (&da)[ITJ], u32x2 &dsv) {
+    auto fetch_to = [&](int64_t i, int j, PubT (&da)[ITQ], u32x2 &dsv) {
         const int64_t ie = i < J.batch ? i : inst;
         int64_t ia = ie; // the instance's matrix (kEnc may read it from a key store)
         if constexpr (MODE == kEnc)
@@ -242,6 +248,12 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
         if constexpr (HALF) {
             // the term's row is polynomial j of the instance (InnerProd) or its only one (polymul)
             da[0] = reinterpret_cast<const u32x2 *>(src)[j * 64 + lane];
+        } else if constexpr (Q3) {
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int pq = (MODE == kMatVec) ? r * L + j : j * L + r;
+                da[r] = reinterpret_cast<const u32x2 *>(src)[pq * 64 + lane];
+            }
         } else {
 #pragma unroll
         for (int it = 0; it < ITJ; it++) {
@@ -274,7 +286,21 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
         uint8_t *const E = slice0 + into * S::BYTES;
         uint8_t *const X = E + S::E_BYTES;
         // ---- public polynomials of term j -> byte-reversed limb blocks E[r][limb][e]
-        if constexpr (HALF) {
+        if constexpr (Q3) {
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const u32x2 v = va[j][r];
+                const uint32_t s0 = shr16x2(v.x, 7), s1 = shr16x2(v.y, 7);
+                const uint32_t plo = perm(v.y, v.x, 0x00020406u) & lo4;
+                const uint32_t phi = perm(s1, s0, 0x00020406u) & hi4;
+                uint8_t *slot = E + r * 2 * kEPoly + 28 - 4 * (lane & 7);
+                const int b4 = lane >> 3;
+                *(uint32_t *)(slot + (b4 + 8) * kEBlock) = plo;
+                *(uint32_t *)(slot + kEPoly + (b4 + 8) * kEBlock) = phi;
+                *(uint32_t *)(slot + b4 * kEBlock) = neg_small(plo);
+                *(uint32_t *)(slot + kEPoly + b4 * kEBlock) = neg_small(phi);
+            }
+        } else if constexpr (HALF) {
             // Lane l holds coefficients 4l..4l+3 (block l >> 3, piece l & 7);
             // coefficient c of a block goes to byte 31 - c, as below.
             const u32x2 v = va[j][0];
@@ -413,7 +439,7 @@ __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t s
 #pragma unroll
                         for (int jj = 0; jj < NT; jj++) {
 #pragma unroll
-                            for (int it = 0; it < ITJ; it++) va[jj][it] = vb[jj][it];
+                            for (int it = 0; it < ITQ; it++) va[jj][it] = vb[jj][it];
                             vs[jj] = vsb[jj];
                         }
                     }
