@@ -178,8 +178,8 @@ template <int L, int MODE> constexpr int kLdsBytes = 4 * 2 * Slice<L, MODE>::BYT
 // PRE2 (16x16x64 only): the wave's range holds at most two instances, and the
 // second one's inputs are loaded with the first's at the start instead of one
 // term ahead (the one-launch KEM's claims, which are too short to hide a load
-// behind one term of work).  QUAD (one public row only): each lane stages
-// four coefficients of a term instead of half the lanes staging eight.
+// behind one term of work).  QUAD: where eight coefficients per lane would
+// leave lanes idle (one public row, or three), each lane stages four.
 template <int L, int MODE, int EPI, int SHAPE = 16, bool STRIDED = false, bool PRE2 = false, bool QUAD = true>
 __device__ __forceinline__ void mfma_body(const Job &J, int64_t first, int64_t stride, uint8_t *const slice0)
 {
