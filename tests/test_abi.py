@@ -92,3 +92,25 @@ def test_reference_named_entry_points_without_gpu():
     lib.saber_enc(9, k, pk, c)
     assert lib.tmvp_host_status() == -1
     assert lib.tmvp_host_release() == 0
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """With no built library there is no fallback: loading it raises, in a
+    fresh process, before any operator can run."""
+    import subprocess
+    import sys
+
+    code = (
+        "import sys\n"
+        "sys.path.insert(0, %r)\n"
+        "import tmvp_amd as T\n"
+        "try:\n"
+        "    T.load_library(%r)\n"
+        "except FileNotFoundError as e:\n"
+        "    print('RAISED', e)\n"
+        "    sys.exit(3)\n"
+        "print('LOADED')\n"
+    ) % (os.path.join(ROOT, "asfand-tmvp_amd"), str(tmp_path / "libtmvp.so"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 3, (r.stdout, r.stderr)
+    assert "RAISED" in r.stdout and "not built" in r.stdout
